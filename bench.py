@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Throughput of the Heston/Bates Monte Carlo path on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload qe_asian_uao|bates_surface|heston_euler]
+
+One step is one pass of the hot path over one batch of synthetic input:
+
+* qe_asian_uao (default): Heston, Andersen QE (gamma1 = gamma2 = 1/2), S0=K=100,
+  T=1, r=0.03, q=0, v0=theta=0.04, kappa=1.5, sigma=0.5, rho=-0.7; 1M paths
+  (500k antithetic pairs) x 365 steps, seed 42; an arithmetic Asian call and an
+  up-and-out call (B=130, daily monitoring), each its own simulation, in one
+  batched call.
+* bates_surface: the same Heston parameters plus lambda=0.5, mu_j=-0.1,
+  sigma_j=0.15; 4M paths x 252 steps per expiry, 8 expiries, 41 call strikes
+  60..140 priced from shared paths.
+* heston_euler: the reference's CPU-sized case, 100k paths x 252 Euler steps.
+
+The value is FP32-path mode path-steps/s (both antithetic siblings count);
+FP64 mode runs the same workload on identical Philox streams and the line
+carries its throughput and the FP32 price error in standard errors.  Kernel
+times come from HIP events the library records on its own stream around the
+path-simulation kernels.  The CPU baseline is the C restatement of the
+reference engine (oracle/) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "options-pricing-engine-rs_amd"))
+
+METRIC = "MC path-steps/s FP32 vs FP64 at matched stderr; p50/p99 request latency"
+HP = dict(v0=0.04, kappa=1.5, theta=0.04, sigma=0.5, rho=-0.7)
+JUMPS = (0.5, -0.1, 0.15)
+EXPIRIES = [0.1, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 2.0]
+STRIKES = [60.0 + 2.0 * i for i in range(41)]
+
+
+def make_workload(ope, wm, name):
+    """Returns (description, config dict, run(precision) -> (prices, ses, kernel_ms), path_steps, flops)."""
+    hp = ope.HestonParams(**HP)
+    if name == "qe_asian_uao":
+        cfg = ope.McConfig(n_paths=1_000_000, n_steps=365, seed=42, scheme=ope.VarianceScheme.QuadraticExponential)
+        reqs = [ope.McRequest(100.0, 1.0, 0.03, 0.0, hp, ope.Payoff.AsianArithmetic(100.0), cfg),
+                ope.McRequest(100.0, 1.0, 0.03, 0.0, hp, ope.Payoff.UpAndOut(100.0, 130.0, 0.0), cfg)]
+
+        def run(precision):
+            res, stats, dev_ms, k_ms = ope.mc_batch(reqs, precision, with_kernel_ms=True)
+            return [r.price for r in res], [r.std_error for r in res], k_ms
+
+        path_steps = 2 * 1_000_000 * 365
+        flops = (wm.flops(1_000_000, 365, "qe", False, 1, "asian") + wm.flops(1_000_000, 365, "qe", False, 1, "up_and_out"))
+        desc = ("Heston QE (gamma1=gamma2=0.5) 1M paths x 365 steps, arithmetic Asian call K=100 and "
+                "up-and-out call K=100 B=130 (one simulation each), seed 42")
+        conf = dict(n_paths=1_000_000, n_steps=365, scheme="QE", payoffs=["asian_call_100", "uao_call_100_130"],
+                    simulations_per_step=2)
+        return desc, conf, run, path_steps, flops
+    if name == "bates_surface":
+        cfg = ope.McConfig(n_paths=4_000_000, n_steps=252, seed=42)
+
+        def run(precision):
+            prices, ses, stats, dev_ms, k_ms = ope.mc_surface(100.0, 0.03, 0.0, hp, JUMPS, 0, EXPIRIES, STRIKES,
+                                                              cfg, precision, with_kernel_ms=True)
+            return list(prices.ravel()), list(ses.ravel()), k_ms
+
+        path_steps = len(EXPIRIES) * 4_000_000 * 252
+        flops = len(EXPIRIES) * wm.flops(4_000_000, 252, "euler", True, len(STRIKES), "european")
+        desc = ("Bates (lambda=0.5, mu_j=-0.1, sigma_j=0.15) Euler 4M paths x 252 steps per expiry, "
+                "8 expiries x 41 call strikes 60..140 from shared paths, seed 42")
+        conf = dict(n_paths=4_000_000, n_steps=252, scheme="Euler", expiries=EXPIRIES, n_strikes=len(STRIKES))
+        return desc, conf, run, path_steps, flops
+    if name == "heston_euler":
+        cfg = ope.McConfig(n_paths=100_000, n_steps=252, seed=42)
+
+        def run(precision):
+            r = ope.mc_price_ex(100.0, 1.0, 0.03, 0.0, hp, ope.Payoff.European(100.0), cfg, None,
+                                ope.McOptions(precision))
+            return [r.result.price], [r.result.std_error], r.kernel_ms
+
+        desc = "Heston European call, full-truncation Euler, 100k paths x 252 steps, seed 42"
+        return (desc, dict(n_paths=100_000, n_steps=252, scheme="Euler"), run, 100_000 * 252,
+                wm.flops(100_000, 252, "euler", False))
+    raise SystemExit(f"unknown workload {name}")
+
+
+def time_mode(run, precision, steps, warmup):
+    for _ in range(warmup):
+        run(precision)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prices, ses, k = run(precision)
+        kms.append(k)
+    wall = time.perf_counter() - t0
+    return prices, ses, wall / steps * 1e3, sum(kms) / len(kms)
+
+
+def cpu_baseline(workload):
+    """C restatement of mc.rs (oracle/), FP64, on this host's cores, bounded sample."""
+    import oracle
+
+    cores = max(1, min(16, os.cpu_count() or 1))
+    hp = oracle.heston(**HP)
+    if workload == "bates_surface":
+        n_paths, n_steps, scheme, pay, jp = 200_000, 252, oracle.EULER, oracle.payoff(), oracle.jumps(*JUMPS)
+        sample = f"Bates Euler European K=100 T=1, {n_paths} paths x {n_steps} steps (one of the 328 surface calls)"
+    elif workload == "heston_euler":
+        n_paths, n_steps, scheme, pay, jp = 100_000, 252, oracle.EULER, oracle.payoff(), None
+        sample = f"Heston Euler European, {n_paths} paths x {n_steps} steps (the full workload)"
+    else:
+        n_paths, n_steps, scheme, pay, jp = 400_000, 365, oracle.QE, oracle.payoff(oracle.ASIAN, 100.0), None
+        sample = f"Heston QE arithmetic Asian K=100, {n_paths} paths x {n_steps} steps"
+    t0 = time.perf_counter()
+    r = oracle.mc_price(100.0, 1.0, 0.03, 0.0, hp, pay, n_paths, n_steps, 42, True, scheme, jp, n_threads=cores)
+    wall = time.perf_counter() - t0
+    return {"value": n_paths * n_steps / wall, "unit": "path-steps/s", "cores": cores, "kind": "port",
+            "sample": sample + f"; {cores} threads, chunk seeding as mc.rs:159-172; {wall:.2f} s wall",
+            "price": r[0], "std_error": r[1]}
+
+
+def load_traffic(kernel_tag):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel_tag)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="qe_asian_uao", choices=["qe_asian_uao", "bates_surface", "heston_euler"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary Bates-surface measurement")
+    ap.add_argument("--profile", action="store_true", help="time FP32-path mode only (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this engine's path has no multi-GPU mode in this build "
+              "(the design shards (request, pair-range) tiles with one 24-byte-per-strike reduction)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import opemc as ope
+    from opemc import work_model as wm
+
+    info = ope.device_info()
+    print(f"# device {info['name']} {info['arch']} CUs={info['compute_units']} clock={info['clock_mhz']} MHz",
+          file=sys.stderr, flush=True)
+    desc, conf, run, path_steps, flops = make_workload(ope, wm, args.workload)
+
+    p32, se32, ms32, k32 = time_mode(run, 1, args.steps, args.warmup)
+    print(f"# fp32 path: {ms32:.3f} ms/step wall, {k32:.3f} ms kernel", file=sys.stderr, flush=True)
+    if args.profile:
+        print(json.dumps({"workload": args.workload, "fp32_ms_per_step": ms32, "fp32_kernel_ms": k32}))
+        return
+    p64, se64, ms64, k64 = time_mode(run, 0, max(3, args.steps // 2), 1)
+    print(f"# fp64: {ms64:.3f} ms/step wall, {k64:.3f} ms kernel", file=sys.stderr, flush=True)
+
+    err_se = [abs(a - b) / s for a, b, s in zip(p32, p64, se64) if s > 0]
+    achieved32 = flops / (k32 * 1e-3) / 1e12
+    achieved64 = flops / (k64 * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": path_steps / (ms32 * 1e-3),
+        "unit": "path-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox4x32-10 draws keyed by seed and pair)",
+        "config": dict(workload=desc, precision="fp32-path, fp64 log-price sum and reduction", **conf),
+        "roofline": {"bound": "valu", "achieved": achieved32, "peak": wm.PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                     "frac": achieved32 / wm.PEAK_TFLOPS["f32"],
+                     "traffic": load_traffic(f"{args.workload}_fp32")},
+        "kernel_ms": k32,
+        "path_steps_per_step": path_steps,
+        "fp64": {"value": path_steps / (ms64 * 1e-3), "ms_per_step": ms64, "kernel_ms": k64,
+                 "roofline": {"bound": "valu", "achieved": achieved64, "peak": wm.PEAK_TFLOPS["f64"],
+                              "unit": "TFLOP/s", "frac": achieved64 / wm.PEAK_TFLOPS["f64"]}},
+        "fp32_over_fp64": ms64 / ms32,
+        "fp32_price_err_max_se": max(err_se) if err_se else None,
+        "prices_fp32": p32[:4],
+        "prices_fp64": p64[:4],
+        "std_errors": se64[:4],
+        "serving": None,
+    }
+    if args.workload == "qe_asian_uao" and not args.no_extra:
+        d2, c2, run2, ps2, fl2 = make_workload(ope, wm, "bates_surface")
+        q32, s32, m32, kk32 = time_mode(run2, 1, 3, 1)
+        q64, s64, m64, kk64 = time_mode(run2, 0, 2, 1)
+        e2 = [abs(a - b) / s for a, b, s in zip(q32, q64, s64) if s > 0]
+        a2 = fl2 / (kk32 * 1e-3) / 1e12
+        line["bates_surface"] = {
+            "workload": d2, "fp32_path_steps_per_s": ps2 / (m32 * 1e-3), "fp64_path_steps_per_s": ps2 / (m64 * 1e-3),
+            "fp32_ms": m32, "fp64_ms": m64, "fp32_over_fp64": m64 / m32, "fp32_price_err_max_se": max(e2),
+            "roofline_fp32": {"bound": "valu", "achieved": a2, "peak": wm.PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                              "frac": a2 / wm.PEAK_TFLOPS["f32"]}}
+        print(f"# bates surface fp32 {m32:.2f} ms fp64 {m64:.2f} ms", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.workload)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

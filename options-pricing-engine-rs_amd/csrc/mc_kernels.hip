@@ -1,0 +1,189 @@
+// Philox-mode pricing kernels, the slot reduction and the generator probes.
+// Built with -ffp-contract=on: a*b+c is fused only inside one source
+// expression, so a request's arithmetic is the same in the specialised and
+// in the mixed-batch kernel, and results do not depend on what shares a
+// launch.
+#include "mc_device.h"
+#include "mc_launch.h"
+
+namespace ope {
+namespace {
+
+struct PhiloxMaker {
+    __device__ __forceinline__ PhiloxSource operator()(const McDesc& d, uint64_t pair) const {
+        PhiloxSource s;
+        s.st = make_stream(d, pair);
+        return s;
+    }
+};
+
+template <int CODE>
+struct Code {
+    static constexpr int kPPT = (CODE & 1) ? 4 : 1;
+    static constexpr int kPayoff = (CODE >> 1) % 3;
+    static constexpr bool kJumps = (((CODE >> 1) / 3) & 1) != 0;
+    static constexpr int kScheme = ((CODE >> 1) / 3) >> 1;
+};
+
+template <class Real, int CODE>
+__device__ __forceinline__ void run_code(const McDesc& d, uint32_t tile, const double* strikes, double2* slots,
+                                         double2* lds) {
+    using C = Code<CODE>;
+    price_tile<Real, C::kScheme, C::kJumps, C::kPayoff, C::kPPT>(d, tile, strikes, slots, PhiloxMaker{},
+                                                                 PathSink{nullptr, nullptr}, lds);
+}
+
+// One block per tile; every request in the launch has config code CODE.
+template <class Real, int CODE>
+__global__ __launch_bounds__(kBlock) void k_tiles_fixed(const McDesc* __restrict__ descs,
+                                                        const uint2* __restrict__ tiles,
+                                                        const double* __restrict__ strikes,
+                                                        double2* __restrict__ slots) {
+    __shared__ double2 lds[kMaxStrikes * 4];
+    const uint2 t = tiles[blockIdx.x];
+    run_code<Real, CODE>(descs[t.x], t.y, strikes, slots, lds);
+}
+
+template <class Real, int CODE>
+__device__ __forceinline__ void dispatch(uint32_t code, const McDesc& d, uint32_t tile, const double* strikes,
+                                         double2* slots, double2* lds) {
+    if constexpr (CODE < (int)kNumCodes) {
+        if (code == (uint32_t)CODE) {
+            run_code<Real, CODE>(d, tile, strikes, slots, lds);
+            return;
+        }
+        dispatch<Real, CODE + 1>(code, d, tile, strikes, slots, lds);
+    }
+}
+
+// One block per tile; each tile's request picks its body (uniform branch).
+template <class Real>
+__global__ __launch_bounds__(kBlock) void k_tiles_mixed(const McDesc* __restrict__ descs,
+                                                        const uint2* __restrict__ tiles,
+                                                        const double* __restrict__ strikes,
+                                                        double2* __restrict__ slots) {
+    __shared__ double2 lds[kMaxStrikes * 4];
+    const uint2 t = tiles[blockIdx.x];
+    const McDesc& d = descs[t.x];
+    dispatch<Real, 0>(config_code(d), d, t.y, strikes, slots, lds);
+}
+
+// Sum one (request, strike) over its tiles in tile order: thread i takes
+// tiles i, i+256, ... then the fixed butterfly / wave order.
+__global__ __launch_bounds__(kBlock) void k_reduce(const McDesc* __restrict__ descs, const uint2* __restrict__ jobs,
+                                                   const double2* __restrict__ slots, double2* __restrict__ out) {
+    __shared__ double2 w[kBlock / 64];
+    const uint2 jb = jobs[blockIdx.x];
+    const McDesc& d = descs[jb.x];
+    const uint32_t nt = d.n_tiles, ns = d.n_strikes;
+    double s = 0.0, q = 0.0;
+    for (uint32_t t = threadIdx.x; t < nt; t += kBlock) {
+        const double2 v = slots[d.slot_off + (uint64_t)t * ns + jb.y];
+        s += v.x;
+        q += v.y;
+    }
+    wave_sum2(s, q);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = make_double2(s, q);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[d.out_off + jb.y] = make_double2(((w[0].x + w[1].x) + w[2].x) + w[3].x, ((w[0].y + w[1].y) + w[2].y) + w[3].y);
+}
+
+__global__ void k_debug_philox(uint32_t k0, uint32_t k1, const uint4* __restrict__ ctr, uint4* __restrict__ out,
+                               uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
+}
+
+// The draws of the pricing loops (sim_pair_f64 / sim_pair_f32), one thread per
+// pair, written [step][pair].
+__global__ void k_debug_draws(const McDesc* __restrict__ desc, uint64_t pair0, uint64_t n_pairs, int fp32,
+                              double* __restrict__ za, double* __restrict__ zb, double* __restrict__ jump,
+                              uint32_t* __restrict__ nj) {
+    const McDesc& d = *desc;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pairs) return;
+    const PairStream st = make_stream(d, pair0 + i);
+    for (uint32_t s = 0; s < d.n_steps; ++s) {
+        uint4 w;
+        uint32_t wa, wb;
+        if (d.jumps) {
+            w = st.call(s, kTagBates);
+            wa = w.x;
+            wb = w.y;
+        } else {
+            w = st.call(s >> 1, kTagHeston);
+            wa = (s & 1) ? w.z : w.x;
+            wb = (s & 1) ? w.w : w.y;
+        }
+        double a, b;
+        if (fp32) {
+            float fa, fb;
+            box_muller_f32(wa, wb, fa, fb);
+            a = fa;
+            b = fb;
+        } else {
+            box_muller_f64(wa, wb, a, b);
+        }
+        uint32_t n = 0;
+        double j = 0.0;
+        if (d.jumps) j = bates_jump(d, w.z, w.w, n);
+        const uint64_t o = (uint64_t)s * n_pairs + i;
+        za[o] = a;
+        zb[o] = b;
+        jump[o] = fp32 ? (double)(float)(j * kLog2e) * kLn2 : j;
+        if (nj) nj[o] = n;
+    }
+}
+
+template <class Real, int CODE = 0>
+hipError_t launch_fixed(uint32_t code, const TileLaunch& L, hipStream_t s) {
+    if constexpr (CODE < (int)kNumCodes) {
+        if (code == (uint32_t)CODE) {
+            hipLaunchKernelGGL((k_tiles_fixed<Real, CODE>), dim3(L.n_tiles), dim3(kBlock), 0, s, L.descs, L.tiles,
+                               L.strikes, L.slots);
+            return hipGetLastError();
+        }
+        return launch_fixed<Real, CODE + 1>(code, L, s);
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_tiles(const TileLaunch& L, hipStream_t s) {
+    if (L.n_tiles == 0) return hipSuccess;
+    if (L.fixed) return L.fp32 ? launch_fixed<float>(L.code, L, s) : launch_fixed<double>(L.code, L, s);
+    if (L.fp32)
+        hipLaunchKernelGGL((k_tiles_mixed<float>), dim3(L.n_tiles), dim3(kBlock), 0, s, L.descs, L.tiles, L.strikes,
+                           L.slots);
+    else
+        hipLaunchKernelGGL((k_tiles_mixed<double>), dim3(L.n_tiles), dim3(kBlock), 0, s, L.descs, L.tiles, L.strikes,
+                           L.slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const McDesc* descs, const uint2* jobs, uint32_t n_jobs, const double2* slots, double2* out,
+                         hipStream_t s) {
+    if (n_jobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce, dim3(n_jobs), dim3(kBlock), 0, s, descs, jobs, slots, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_philox(uint64_t seed, const uint4* ctr, uint4* out, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_debug_philox, dim3((n + 255) / 256), dim3(256), 0, s, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), ctr, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_draws(const McDesc* desc, uint64_t pair0, uint64_t n_pairs, int fp32, double* za, double* zb,
+                              double* jump, uint32_t* n_jumps, hipStream_t s) {
+    if (n_pairs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_debug_draws, dim3((unsigned)((n_pairs + 255) / 256)), dim3(256), 0, s, desc, pair0, n_pairs,
+                       fp32, za, zb, jump, n_jumps);
+    return hipGetLastError();
+}
+
+}  // namespace ope

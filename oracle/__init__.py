@@ -1,0 +1,299 @@
+"""ORACLE -- TEST INFRASTRUCTURE ONLY.
+
+Python access to the CPU oracle:
+
+* ``liboracle.so`` (``mc_ref.c``, ``analytic.c``): the C restatement of the
+  reference engine ``src/mc.rs`` (draw stream, ``run_path``, ``mc_price``) and
+  of its analytic Heston/Bates/BSM pricers (``src/heston.rs``,
+  ``src/bates.rs``, ``src/bsm.rs``).  File:line citations are in those
+  sources.
+* ``philox_*``: a numpy restatement of the GPU engine's own counter-based
+  Philox4x32-10 draw specification (the reference has no counterpart: its
+  draws come from rand's SmallRng).  It is written from the Random123 paper
+  (Salmon et al., SC'11) and pinned by that paper's known-answer vectors.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package, and only as the checker or as the timed CPU
+baseline.  The product library never calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+EUROPEAN, ASIAN, UP_AND_OUT = 0, 1, 2
+CALL, PUT = 0, 1
+EULER, QE = 0, 1
+
+
+class Heston(C.Structure):
+    _fields_ = [("v0", C.c_double), ("kappa", C.c_double), ("theta", C.c_double),
+                ("sigma", C.c_double), ("rho", C.c_double)]
+
+
+class PayoffS(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("opt_type", C.c_int32), ("strike", C.c_double),
+                ("barrier", C.c_double), ("rebate", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [("n_paths", C.c_uint64), ("n_steps", C.c_uint64), ("seed", C.c_uint64),
+                ("antithetic", C.c_int32), ("scheme", C.c_int32)]
+
+
+class Jumps(C.Structure):
+    _fields_ = [("has_jumps", C.c_int32), ("lambda_", C.c_double), ("mu_j", C.c_double),
+                ("sigma_j", C.c_double)]
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load liboracle.so (built by ``make -C oracle`` / ``__graft_entry__.build()``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+    L = C.CDLL(LIB_PATH)
+    L.ora_splitmix64.restype = C.c_uint64
+    L.ora_splitmix64.argtypes = [C.c_uint64]
+    L.ora_smallrng_seed.argtypes = [C.c_uint64, _u64p]
+    L.ora_xoshiro_next_n.argtypes = [_u64p, C.c_uint64, _u64p]
+    L.ora_pcg32_canonical.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, _u32p]
+    L.ora_gen_f64_n.argtypes = [C.c_uint64, C.c_uint64, _dp]
+    L.ora_gen_range_n.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_uint64, _dp]
+    L.ora_box_muller_n.argtypes = [C.c_uint64, C.c_uint64, _dp]
+    L.ora_sample_poisson_n.argtypes = [C.c_uint64, C.c_double, C.c_uint64, _u32p]
+    L.ora_qe_variance_step_n.argtypes = [_dp, _dp, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, _dp]
+    L.ora_ncdf_n.argtypes = [_dp, C.c_uint64, _dp]
+    L.ora_ncdf.restype = C.c_double
+    L.ora_ncdf.argtypes = [C.c_double]
+    L.ora_mc_price.restype = C.c_int
+    L.ora_mc_price.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(Heston),
+                               C.POINTER(Jumps), C.POINTER(PayoffS), C.POINTER(Config), C.c_uint64,
+                               C.c_int, _dp, C.POINTER(C.c_uint64)]
+    L.ora_gen_draws.restype = C.c_int
+    L.ora_gen_draws.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double,
+                                C.POINTER(Jumps), _dp, _dp, _dp]
+    L.ora_run_paths.restype = C.c_int
+    L.ora_run_paths.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(Heston),
+                                C.POINTER(Jumps), C.POINTER(PayoffS), C.c_uint64, C.c_int, C.c_int,
+                                C.c_uint64, _dp, _dp, _dp, _dp, _dp]
+    L.ora_heston_price.restype = C.c_double
+    L.ora_heston_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int]
+    L.ora_bates_price.restype = C.c_double
+    L.ora_bates_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston)] + [C.c_double] * 3 + [C.c_int]
+    L.ora_bsm_price.restype = C.c_double
+    L.ora_bsm_price.argtypes = [C.c_double] * 6 + [C.c_int]
+    L.ora_merton_price.restype = C.c_double
+    L.ora_merton_price.argtypes = [C.c_double] * 9 + [C.c_int]
+    _lib = L
+    return L
+
+
+# --------------------------------------------------------------- helpers
+
+def heston(v0, kappa, theta, sigma, rho):
+    return Heston(v0, kappa, theta, sigma, rho)
+
+
+def payoff(kind=EUROPEAN, strike=100.0, opt_type=CALL, barrier=0.0, rebate=0.0):
+    return PayoffS(kind, opt_type, strike, barrier, rebate)
+
+
+def jumps(lam=None, mu_j=0.0, sigma_j=0.0):
+    if lam is None:
+        return Jumps(0, 0.0, 0.0, 0.0)
+    return Jumps(1, lam, mu_j, sigma_j)
+
+
+def mc_price(spot, expiry, rate, div_yield, hp, pay, n_paths, n_steps, seed, antithetic=True,
+             scheme=EULER, jump=None, n_threads=1, workers=None):
+    """mc_heston / mc_bates (mc.rs:98-113, 148-207) with rayon's thread count
+    fixed to ``n_threads``.  Returns (price, std_error, sum, sum_sq, count)."""
+    L = lib()
+    cfg = Config(n_paths, n_steps, seed, 1 if antithetic else 0, scheme)
+    out = np.zeros(4)
+    cnt = C.c_uint64(0)
+    j = jump if jump is not None else jumps()
+    w = n_threads if workers is None else workers
+    rc = L.ora_mc_price(spot, expiry, rate, div_yield, C.byref(hp), C.byref(j), C.byref(pay),
+                        C.byref(cfg), n_threads, w, out, C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError("ora_mc_price failed")
+    return float(out[0]), float(out[1]), float(out[2]), float(out[3]), int(cnt.value)
+
+
+def gen_draws(seed, n_pairs, n_steps, expiry, jump=None, n_threads=1):
+    """Draws mc_price would consume, as three [n_steps, n_pairs] arrays."""
+    L = lib()
+    za = np.zeros(n_steps * n_pairs)
+    zb = np.zeros(n_steps * n_pairs)
+    jp = np.zeros(n_steps * n_pairs)
+    j = jump if jump is not None else jumps()
+    L.ora_gen_draws(seed, n_pairs, n_steps, n_threads, expiry, C.byref(j), za, zb, jp)
+    return (za.reshape(n_steps, n_pairs), zb.reshape(n_steps, n_pairs), jp.reshape(n_steps, n_pairs))
+
+
+def run_paths(spot, expiry, rate, div_yield, hp, pay, n_steps, scheme, antithetic, za, zb, jump_sum,
+              jump=None):
+    """run_path (mc.rs:210-278) per pair on given [n_steps, n_pairs] draws.
+    Returns payoff [n_pairs, 2] and state [n_pairs, 2, 4] = (ln_s, v, sum, max)."""
+    L = lib()
+    n_pairs = za.shape[1]
+    za = np.ascontiguousarray(za, dtype=np.float64).ravel()
+    zb = np.ascontiguousarray(zb, dtype=np.float64).ravel()
+    jp = np.ascontiguousarray(jump_sum, dtype=np.float64).ravel()
+    pay_out = np.full(n_pairs * 2, np.nan)
+    st_out = np.full(n_pairs * 8, np.nan)
+    j = jump if jump is not None else jumps()
+    L.ora_run_paths(spot, expiry, rate, div_yield, C.byref(hp), C.byref(j), C.byref(pay), n_steps,
+                    scheme, 1 if antithetic else 0, n_pairs, za, zb, jp, pay_out, st_out)
+    return pay_out.reshape(n_pairs, 2), st_out.reshape(n_pairs, 2, 4)
+
+
+def heston_price(s, k, t, r, q, hp, opt_type=CALL):
+    return lib().ora_heston_price(s, k, t, r, q, C.byref(hp), opt_type)
+
+
+def bates_price(s, k, t, r, q, hp, lam, mu_j, sigma_j, opt_type=CALL):
+    return lib().ora_bates_price(s, k, t, r, q, C.byref(hp), lam, mu_j, sigma_j, opt_type)
+
+
+def bsm_price(s, k, t, r, q, vol, opt_type=CALL):
+    return lib().ora_bsm_price(s, k, t, r, q, vol, opt_type)
+
+
+def merton_price(s, k, t, r, q, var, lam, mu_j, sigma_j, opt_type=CALL):
+    return lib().ora_merton_price(s, k, t, r, q, var, lam, mu_j, sigma_j, opt_type)
+
+
+def qe_variance_step(v_t, z, kappa, theta, sigma, dt):
+    v_t = np.ascontiguousarray(v_t, dtype=np.float64)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    v_t, z = np.broadcast_arrays(v_t, z)
+    v_t = np.ascontiguousarray(v_t)
+    z = np.ascontiguousarray(z)
+    out = np.zeros(v_t.size)
+    lib().ora_qe_variance_step_n(v_t.ravel(), z.ravel(), v_t.size, kappa, theta, sigma, dt, out)
+    return out.reshape(v_t.shape)
+
+
+def ncdf(x):
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.zeros(x.size)
+    lib().ora_ncdf_n(x, x.size, out)
+    return out
+
+
+# ------------------------------------------- Philox4x32-10 stream oracle
+
+M32 = 0xFFFFFFFF
+PHILOX_M0, PHILOX_M1 = 0xD2511F53, 0xCD9E8D57
+PHILOX_W0, PHILOX_W1 = 0x9E3779B9, 0xBB67AE85
+TAG_HESTON, TAG_BATES = 0, 1
+
+
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Philox4x32 with 10 rounds (Salmon et al. 2011).  Counters are uint64
+    arrays holding 32-bit values; the key is two Python ints."""
+    c0, c1, c2, c3 = (np.asarray(c, dtype=np.uint64) & np.uint64(M32) for c in (c0, c1, c2, c3))
+    k0 = int(k0) & M32
+    k1 = int(k1) & M32
+    m0, m1, mask = np.uint64(PHILOX_M0), np.uint64(PHILOX_M1), np.uint64(M32)
+    for _ in range(10):
+        p0 = m0 * c0
+        p1 = m1 * c2
+        hi0, lo0 = p0 >> np.uint64(32), p0 & mask
+        hi1, lo1 = p1 >> np.uint64(32), p1 & mask
+        c0, c1, c2, c3 = (hi1 ^ c1 ^ np.uint64(k0), lo1, hi0 ^ c3 ^ np.uint64(k1), lo0)
+        k0 = (k0 + PHILOX_W0) & M32
+        k1 = (k1 + PHILOX_W1) & M32
+    return c0, c1, c2, c3
+
+
+def poisson_threshold0(mu):
+    """Integer threshold T with (w < T) <=> ((w + 0.5) 2^-32 < e^-mu)."""
+    f0 = math.exp(-mu)
+    return int(math.ceil(f0 * 4294967296.0 - 0.5))
+
+
+def poisson_from_word(w, mu, kmax=1024):
+    """Inverse-CDF Poisson count of the uniform (w + 0.5) 2^-32 (FP64 CDF)."""
+    out = np.zeros(np.shape(w), dtype=np.int64)
+    if mu <= 0.0:
+        return out
+    thr = poisson_threshold0(mu)
+    e0 = math.exp(-mu)
+    flat_w = np.asarray(w, dtype=np.uint64).ravel()
+    flat = out.ravel()
+    for idx in np.nonzero(flat_w >= np.uint64(thr))[0]:
+        u = (float(flat_w[idx]) + 0.5) * 2.0 ** -32
+        k, p, F = 0, e0, e0
+        while u >= F and k < kmax:
+            k += 1
+            p = p * mu / k
+            F = F + p
+        flat[idx] = k
+    return flat.reshape(np.shape(w))
+
+
+def philox_draws(seed, pair_ids, n_steps, dt=None, lam=0.0, mu_j=0.0, sigma_j=0.0):
+    """The GPU engine's Philox-mode draws, FP64, for the given pair indices.
+
+    Heston (lam == 0): step s uses Philox call s >> 1 with tag 0, words (0,1)
+    for even s and (2,3) for odd s, mapped by Box-Muller.  Bates (lam > 0):
+    step s uses call s with tag 1; words 0,1 -> Box-Muller normals, word 2 ->
+    Poisson count of mean lam*dt by inverse CDF, word 3 -> the jump normal by
+    the inverse normal CDF.  Counter = (call, pair lo, pair hi, tag), key =
+    (seed lo, seed hi).  Returns za, zb, jump_sum, n as [n_steps, n_pairs].
+    """
+    from scipy.special import ndtri
+
+    pair_ids = np.asarray(pair_ids, dtype=np.uint64)
+    npairs = pair_ids.size
+    k0, k1 = int(seed) & M32, (int(seed) >> 32) & M32
+    plo = pair_ids & np.uint64(M32)
+    phi = pair_ids >> np.uint64(32)
+    za = np.zeros((n_steps, npairs))
+    zb = np.zeros((n_steps, npairs))
+    js = np.zeros((n_steps, npairs))
+    nn = np.zeros((n_steps, npairs), dtype=np.int64)
+    bates = lam > 0.0
+    words_cache = {}
+    for s in range(n_steps):
+        call = s if bates else s >> 1
+        if call not in words_cache:
+            words_cache.clear()
+            words_cache[call] = philox4x32_10(np.full(npairs, call, dtype=np.uint64), plo, phi,
+                                              np.full(npairs, TAG_BATES if bates else TAG_HESTON,
+                                                      dtype=np.uint64), k0, k1)
+        w = words_cache[call]
+        if bates or (s & 1) == 0:
+            wa, wb = w[0], w[1]
+        else:
+            wa, wb = w[2], w[3]
+        u1 = (wa.astype(np.float64) + 0.5) * 2.0 ** -32
+        u2 = wb.astype(np.float64) * 2.0 ** -32
+        r = np.sqrt(-2.0 * np.log(u1))
+        za[s] = r * np.cos(2.0 * np.pi * u2)
+        zb[s] = r * np.sin(2.0 * np.pi * u2)
+        if bates:
+            mu = lam * dt
+            n = poisson_from_word(w[2], mu)
+            nn[s] = n
+            zj = ndtri((w[3].astype(np.float64) + 0.5) * 2.0 ** -32)
+            js[s] = np.where(n > 0, mu_j * n + sigma_j * np.sqrt(n) * zj, 0.0)
+    return za, zb, js, nn

@@ -1,0 +1,262 @@
+/*
+ * ORACLE -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Statistical oracle for the Monte Carlo path: C restatement of the
+ * reference's analytic pricers, which its own MC tests compare against
+ * (mc.rs:393-429):
+ *   heston_price   heston.rs:61-72, heston_call 82-99, cf_integrand 101-109,
+ *                  stable_cf (Albrecher form) 113-127, fast_csqrt 155-175,
+ *                  gk15_panel 180-187, adaptive_gk 193-212, gk_integrate 220-229
+ *   bates_price    bates.rs:29-39, bates_call 49-63, bates_integrand 65-73, jump_cf 77-83
+ *   bsm_price      bsm.rs:7-12, 55-60
+ * plus the textbook Merton (1976) jump-diffusion series, which has no
+ * reference counterpart and pins the Bates MC at zero vol-of-vol.
+ *
+ * Complex arithmetic follows num-complex 0.4 (naive product and quotient,
+ * exp via from_polar, ln via (ln|z|, atan2)).  Compiled -ffp-contract=off.
+ */
+#include "oracle.h"
+
+#include <math.h>
+
+typedef struct { double re, im; } cpx;
+
+static inline cpx C(double re, double im) { cpx z = {re, im}; return z; }
+static inline cpx cadd(cpx a, cpx b) { return C(a.re + b.re, a.im + b.im); }
+static inline cpx csub(cpx a, cpx b) { return C(a.re - b.re, a.im - b.im); }
+static inline cpx cmul(cpx a, cpx b) { return C(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static inline cpx cdiv(cpx a, cpx b) {
+    double n = b.re * b.re + b.im * b.im;
+    double re = a.re * b.re + a.im * b.im;
+    double im = a.im * b.re - a.re * b.im;
+    return C(re / n, im / n);
+}
+static inline cpx cscale(cpx a, double s) { return C(a.re * s, a.im * s); }       /* Complex * f64 */
+static inline cpx cradd(cpx a, double s) { return C(a.re + s, a.im); }            /* Complex + f64 */
+static inline cpx crsub_l(double s, cpx a) { return C(s - a.re, -a.im); }         /* f64 - Complex */
+static inline cpx cexp_(cpx z) { double e = exp(z.re); return C(e * cos(z.im), e * sin(z.im)); }
+static inline cpx clog_(cpx z) { return C(log(hypot(z.re, z.im)), atan2(z.im, z.re)); }
+
+/* heston.rs:155-175 */
+static cpx fast_csqrt(cpx z) {
+    double a = z.re, b = z.im;
+    if (b == 0.0) {
+        if (a >= 0.0) return C(sqrt(a), b);
+        double m = sqrt(-a);
+        return C(0.0, signbit(b) ? -m : m);
+    }
+    double r = hypot(a, b);
+    if (a >= 0.0) {
+        double re_out = sqrt((r + a) / 2.0);
+        double im_out = b / (2.0 * re_out);
+        return C(re_out, im_out);
+    } else {
+        double im_out = copysign(sqrt((r - a) / 2.0), b);
+        double re_out = b / (2.0 * im_out);
+        return C(re_out, im_out);
+    }
+}
+
+/* heston.rs:113-127 */
+static cpx stable_cf(cpx phi, double t, double r, const ora_heston *p) {
+    const cpx I = C(0.0, 1.0);
+    double v0 = p->v0, kappa = p->kappa, theta = p->theta, sigma = p->sigma, rho = p->rho;
+    cpx xi = crsub_l(kappa, cmul(cscale(phi, rho * sigma), I));
+    cpx d = fast_csqrt(cadd(cmul(xi, xi), cmul(cscale(phi, sigma * sigma), cadd(phi, I))));
+    cpx g = cdiv(csub(xi, d), cadd(xi, d));
+    cpx edt = cexp_(cscale(C(-d.re, -d.im), t));
+    cpx a = cdiv(cradd(cmul(g, edt), -1.0), cradd(g, -1.0));
+    cpx c = cscale(csub(cscale(csub(xi, d), t), cscale(clog_(a), 2.0)), kappa * theta / (sigma * sigma));
+    cpx dd = cdiv(cmul(cscale(csub(xi, d), v0), crsub_l(1.0, edt)),
+                  cscale(crsub_l(1.0, cmul(g, edt)), sigma * sigma));
+    cpx rpi = cscale(cmul(cscale(phi, r), I), t);
+    return cexp_(cadd(cadd(rpi, c), dd));
+}
+
+/* bates.rs:77-83 */
+static cpx jump_cf(cpx phi, double t, double lambda, double mu_j, double sigma_j) {
+    const cpx I = C(0.0, 1.0);
+    double comp = exp(mu_j + 0.5 * sigma_j * sigma_j) - 1.0;
+    cpx jump = cexp_(csub(cscale(cmul(phi, I), mu_j),
+                          cscale(cscale(cmul(cscale(phi, 0.5), phi), sigma_j), sigma_j)));
+    cpx inner = csub(cradd(jump, -1.0), cscale(cmul(I, phi), comp));
+    return cexp_(cscale(inner, lambda * t));
+}
+
+typedef struct {
+    double x, t, r;
+    const ora_heston *p;
+    int is_p1;
+    int has_norm;
+    cpx norm;
+    int bates;
+    double lambda, mu_j, sigma_j;
+} integrand_ctx;
+
+/* heston.rs:101-109 and bates.rs:65-73 */
+static double integrand(const integrand_ctx *c, double u) {
+    cpx phi = c->is_p1 ? C(u, -1.0) : C(u, 0.0);
+    cpx cf = stable_cf(phi, c->t, c->r, c->p);
+    if (c->bates) cf = cmul(cf, jump_cf(phi, c->t, c->lambda, c->mu_j, c->sigma_j));
+    if (c->has_norm) cf = cdiv(cf, c->norm);
+    cpx num = cmul(cexp_(C(0.0, u * c->x)), cf);
+    return cdiv(num, C(0.0, u)).re;
+}
+
+/* heston.rs:28-59: GK15 nodes and weights, G7 subset. */
+static const double GK_NODES[15] = {
+    0.0,
+    0.2077849550078985, -0.2077849550078985,
+    0.4058451513773972, -0.4058451513773972,
+    0.5860872354676911, -0.5860872354676911,
+    0.7415311855993945, -0.7415311855993945,
+    0.8648644233597691, -0.8648644233597691,
+    0.9491079123427585, -0.9491079123427585,
+    0.9914553711208126, -0.9914553711208126,
+};
+static const double GK_WEIGHTS[15] = {
+    0.2094821410847278,
+    0.2044329400752989, 0.2044329400752989,
+    0.1903505780647854, 0.1903505780647854,
+    0.1690047266392679, 0.1690047266392679,
+    0.1406532597155259, 0.1406532597155259,
+    0.1047900103222502, 0.1047900103222502,
+    0.0630920926299786, 0.0630920926299786,
+    0.0229353220105292, 0.0229353220105292,
+};
+static const int G7_IDX[7] = {0, 3, 4, 7, 8, 11, 12};
+static const double G7_WEIGHTS[7] = {
+    0.4179591836734694,
+    0.3818300505051189, 0.3818300505051189,
+    0.2797053914892766, 0.2797053914892766,
+    0.1294849661688697, 0.1294849661688697,
+};
+
+/* heston.rs:220-229: the u = (1-t)/t map of [0, inf) onto [0, 1]. */
+static double mapped(const integrand_ctx *c, double t) {
+    if (t <= 0.0) return 0.0;
+    double u = (1.0 - t) / t;
+    if (u < 1e-12) return 0.0;
+    double v = integrand(c, u) / (t * t);
+    return isfinite(v) ? v : 0.0;
+}
+
+/* heston.rs:180-187 */
+static void gk15_panel(const integrand_ctx *c, double a, double b, double *kout, double *eout) {
+    double cc = 0.5 * (a + b), h = 0.5 * (b - a);
+    double fv[15];
+    for (int i = 0; i < 15; ++i) fv[i] = mapped(c, cc + h * GK_NODES[i]);
+    double k = 0.0, g = 0.0;
+    for (int i = 0; i < 15; ++i) k += GK_WEIGHTS[i] * fv[i];
+    for (int j = 0; j < 7; ++j) g += G7_WEIGHTS[j] * fv[G7_IDX[j]];
+    *kout = k * h;
+    *eout = fabs(k - g) * h;
+}
+
+/* heston.rs:193-212: bisect the panel with the largest error (the last such
+ * panel on ties, as Iterator::max_by does) until the error sum <= tol or 200
+ * panels; swap_remove semantics kept. */
+static double adaptive_gk(const integrand_ctx *c, double a, double b, double tol) {
+    enum { MAX_PANELS = 200 };
+    double pe[MAX_PANELS + 2], pk[MAX_PANELS + 2], pa[MAX_PANELS + 2], pb[MAX_PANELS + 2];
+    int n = 0;
+    double k0, e0;
+    gk15_panel(c, a, b, &k0, &e0);
+    pe[0] = e0; pk[0] = k0; pa[0] = a; pb[0] = b; n = 1;
+    double total = k0, err = e0;
+    while (err > tol && n < MAX_PANELS) {
+        int w = 0;
+        for (int i = 1; i < n; ++i) if (pe[i] >= pe[w]) w = i;
+        double ew = pe[w], kw = pk[w], aw = pa[w], bw = pb[w];
+        /* swap_remove */
+        pe[w] = pe[n - 1]; pk[w] = pk[n - 1]; pa[w] = pa[n - 1]; pb[w] = pb[n - 1];
+        n -= 1;
+        double m = 0.5 * (aw + bw);
+        double kl, el, kr, er;
+        gk15_panel(c, aw, m, &kl, &el);
+        gk15_panel(c, m, bw, &kr, &er);
+        total += kl + kr - kw;
+        err += el + er - ew;
+        pe[n] = el; pk[n] = kl; pa[n] = aw; pb[n] = m; n++;
+        pe[n] = er; pk[n] = kr; pa[n] = m; pb[n] = bw; n++;
+    }
+    return total;
+}
+
+static double cf_call(double s, double k, double t, double r, double q, const ora_heston *p, int bates,
+                      double lambda, double mu_j, double sigma_j) {
+    integrand_ctx c;
+    c.x = log(s / k);
+    c.t = t;
+    c.r = r;
+    c.p = p;
+    c.bates = bates;
+    c.lambda = lambda;
+    c.mu_j = mu_j;
+    c.sigma_j = sigma_j;
+    cpx mi = C(0.0, -1.0);
+    cpx cf_mi = stable_cf(mi, t, r, p);
+    if (bates) cf_mi = cmul(cf_mi, jump_cf(mi, t, lambda, mu_j, sigma_j));
+    c.is_p1 = 1; c.has_norm = 1; c.norm = cf_mi;
+    double i1 = adaptive_gk(&c, 0.0, 1.0, 1e-8);
+    c.is_p1 = 0; c.has_norm = 0;
+    double i2 = adaptive_gk(&c, 0.0, 1.0, 1e-8);
+    const double PI = 3.141592653589793;
+    double p1 = 0.5 + i1 / PI;
+    double p2 = 0.5 + i2 / PI;
+    double v = s * exp(-q * t) * p1 - k * exp(-r * t) * p2;
+    return v > 0.0 ? v : 0.0;
+}
+
+/* heston.rs:61-72 (put by parity) */
+double ora_heston_price(double s, double k, double t, double r, double q, const ora_heston *p, int opt_type) {
+    double call = cf_call(s, k, t, r, q, p, 0, 0, 0, 0);
+    if (opt_type == ORA_CALL) return call;
+    return call - s * exp(-q * t) + k * exp(-r * t);
+}
+
+/* bates.rs:29-39 */
+double ora_bates_price(double s, double k, double t, double r, double q, const ora_heston *p, double lambda,
+                       double mu_j, double sigma_j, int opt_type) {
+    double call = cf_call(s, k, t, r, q, p, 1, lambda, mu_j, sigma_j);
+    if (opt_type == ORA_CALL) return call;
+    return call - s * exp(-q * t) + k * exp(-r * t);
+}
+
+double ora_ncdf(double x);
+
+/* bsm.rs:7-12, 55-60 */
+double ora_bsm_price(double s, double k, double t, double r, double q, double vol, int opt_type) {
+    double vt = vol * sqrt(t);
+    double d1 = (log(s / k) + (r - q + 0.5 * vol * vol) * t) / vt;
+    double d2 = d1 - vt;
+    double phi = opt_type == ORA_CALL ? 1.0 : -1.0;
+    return phi * (s * exp(-q * t) * ora_ncdf(phi * d1) - k * exp(-r * t) * ora_ncdf(phi * d2));
+}
+
+/* Merton (1976) lognormal jump-diffusion, constant diffusion variance `var`:
+ * conditional on n jumps ln S_T is normal with forward
+ * F_n = S exp((r_n - q) T), r_n = r - lambda kbar + n (mu_j + sigma_j^2/2) / T,
+ * and total variance var T + n sigma_j^2, so the call is
+ * sum_n Poisson(n; lambda T) e^{-rT} (F_n N(d1) - K N(d2)). */
+double ora_merton_price(double s, double k, double t, double r, double q, double var, double lambda,
+                        double mu_j, double sigma_j, int opt_type) {
+    double kbar = exp(mu_j + 0.5 * sigma_j * sigma_j) - 1.0;
+    double lp = lambda * t;
+    double total = 0.0, w = exp(-lp);
+    for (int n = 0; n < 200; ++n) {
+        if (n > 0) w *= lp / n;
+        double rn = r - lambda * kbar + n * (mu_j + 0.5 * sigma_j * sigma_j) / t;
+        double vn = sqrt(var + n * sigma_j * sigma_j / t);
+        /* BSM with rate rn for the forward, discounted at r */
+        double fwd = s * exp((rn - q) * t);
+        double sd = vn * sqrt(t);
+        double d1 = (log(fwd / k) + 0.5 * sd * sd) / sd;
+        double d2 = d1 - sd;
+        double phi = opt_type == ORA_CALL ? 1.0 : -1.0;
+        double price = exp(-r * t) * phi * (fwd * ora_ncdf(phi * d1) - k * ora_ncdf(phi * d2));
+        total += w * price;
+        if (n > 10 && w < 1e-18) break;
+    }
+    return total;
+}

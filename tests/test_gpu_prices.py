@@ -1,0 +1,179 @@
+"""Statistical oracle and exact identities for the Philox-mode GPU engine.
+
+* The reference's MC tests (mc.rs:370-455) at the same parameters, in FP64
+  and FP32-path mode.
+* GPU vs CPU oracle (restatement of mc.rs) at equal path counts: within 3
+  combined standard errors.
+* Identities that hold exactly: martingale and Asian-mean closed forms
+  (within SE), knock-out at t=0, barrier at infinity == European,
+  mc_bates(lambda=0) == mc_heston, determinism.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REF = dict(v0=0.04, kappa=2.0, theta=0.04, sigma=0.3, rho=-0.7)   # mc.rs:351-353
+BASE = dict(v0=0.04, kappa=1.5, theta=0.04, sigma=0.5, rho=-0.7)  # BASELINE params
+PRECISIONS = [0, 1]
+
+
+def small_cfg(ope, scheme=0):
+    return ope.McConfig(n_paths=60_000, n_steps=100, seed=7, antithetic=True, scheme=scheme)  # mc.rs:355-357
+
+
+def run(ope, precision, payoff, cfg, hp=REF, jumps=None, spot=100.0, expiry=1.0, rate=0.05, q=0.0):
+    print(f"gpu price precision={precision} payoff={payoff} cfg={cfg} jumps={jumps}", flush=True)
+    return ope.mc_price_ex(spot, expiry, rate, q, ope.HestonParams(**hp), payoff, cfg, jumps,
+                           ope.McOptions(precision))
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_mc_heston_matches_analytic_european(ope, ora, precision, scheme):
+    analytic = ora.heston_price(100, 100, 1, 0.05, 0, ora.heston(**REF))
+    r = run(ope, precision, ope.Payoff.European(100.0), small_cfg(ope, scheme)).result
+    assert abs(r.price - analytic) / r.std_error < 4
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_mc_bates_matches_analytic_european(ope, ora, precision):
+    analytic = ora.bates_price(100, 100, 1, 0.05, 0, ora.heston(**REF), 0.5, -0.1, 0.15)
+    r = run(ope, precision, ope.Payoff.European(100.0), small_cfg(ope), jumps=(0.5, -0.1, 0.15)).result
+    assert abs(r.price - analytic) / r.std_error < 4
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_asian_and_barrier_orderings(ope, precision):
+    euro = run(ope, precision, ope.Payoff.European(100.0), small_cfg(ope)).result.price
+    asian = run(ope, precision, ope.Payoff.AsianArithmetic(100.0), small_cfg(ope)).result.price
+    uao = run(ope, precision, ope.Payoff.UpAndOut(100.0, 130.0), small_cfg(ope)).result.price
+    assert asian < euro          # mc.rs:434-441
+    assert 0.0 <= uao < euro     # mc.rs:446-455
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_qe_reduces_bias_in_feller_violating_regime(ope, ora, precision):
+    hp = dict(v0=0.04, kappa=5.0, theta=0.04, sigma=1.2, rho=-0.7)   # mc.rs:370-391
+    analytic = ora.heston_price(100, 100, 1, 0.05, 0, ora.heston(**hp))
+    e = run(ope, precision, ope.Payoff.European(100.0),
+            ope.McConfig(n_paths=200_000, n_steps=8, seed=42, scheme=0), hp=hp).result
+    q = run(ope, precision, ope.Payoff.European(100.0),
+            ope.McConfig(n_paths=200_000, n_steps=8, seed=42, scheme=1), hp=hp).result
+    euler_err, qe_err = abs(e.price - analytic), abs(q.price - analytic)
+    assert qe_err < euler_err / 5
+    assert qe_err < 10 * max(q.std_error, 0.05)
+
+
+STAT_CASES = [
+    # (name, scheme, payoff args, jumps, n_paths, n_steps, rate, q)
+    ("cfg1_heston_euler_euro", 0, ("E", 100.0), None, 100_000, 252, 0.03, 0.0),
+    ("qe_asian", 1, ("A", 100.0), None, 100_000, 365, 0.03, 0.0),
+    ("qe_uao", 1, ("U", 100.0, 130.0), None, 100_000, 365, 0.03, 0.0),
+    ("bates_euler_euro_put", 0, ("P", 95.0), (0.5, -0.1, 0.15), 100_000, 252, 0.03, 0.01),
+    ("bates_qe_uao", 1, ("U", 100.0, 120.0), (1.0, -0.05, 0.2), 60_000, 100, 0.03, 0.0),
+]
+
+
+def _payoff(ope, spec):
+    if spec[0] == "E":
+        return ope.Payoff.European(spec[1])
+    if spec[0] == "P":
+        return ope.Payoff.European(spec[1], ope.OptionType.Put)
+    if spec[0] == "A":
+        return ope.Payoff.AsianArithmetic(spec[1])
+    return ope.Payoff.UpAndOut(spec[1], spec[2])
+
+
+def _ora_payoff(ora, spec):
+    if spec[0] == "E":
+        return ora.payoff(ora.EUROPEAN, spec[1])
+    if spec[0] == "P":
+        return ora.payoff(ora.EUROPEAN, spec[1], ora.PUT)
+    if spec[0] == "A":
+        return ora.payoff(ora.ASIAN, spec[1])
+    return ora.payoff(ora.UP_AND_OUT, spec[1], barrier=spec[2])
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("case", STAT_CASES, ids=[c[0] for c in STAT_CASES])
+def test_gpu_matches_cpu_oracle_within_3_se(ope, ora, precision, case):
+    name, scheme, pspec, jumps, n_paths, n_steps, rate, q = case
+    cfg = ope.McConfig(n_paths=n_paths, n_steps=n_steps, seed=42, scheme=scheme)
+    g = run(ope, precision, _payoff(ope, pspec), cfg, hp=BASE, jumps=jumps, rate=rate, q=q).result
+    jp = ora.jumps(*jumps) if jumps else None
+    c = ora.mc_price(100.0, 1.0, rate, q, ora.heston(**BASE), _ora_payoff(ora, pspec), n_paths, n_steps, 42,
+                     True, scheme, jp, n_threads=8)
+    assert abs(g.price - c[0]) < 3 * math.hypot(g.std_error, c[1]), (g, c[:2])
+    assert 0.8 < g.std_error / c[1] < 1.25
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("scheme", [0, 1])
+@pytest.mark.parametrize("jumps", [None, (0.5, -0.1, 0.15)])
+def test_discounted_spot_is_a_martingale(ope, precision, scheme, jumps):
+    # E[e^{-rT} S_T] = S0 e^{-qT}: pins the drift, including the jump compensator (mc.rs:155-157)
+    cfg = ope.McConfig(n_paths=400_000, n_steps=50, seed=3, scheme=scheme)
+    r = run(ope, precision, ope.Payoff.European(0.0), cfg, hp=BASE, jumps=jumps, rate=0.03, q=0.02).result
+    assert abs(r.price - 100.0 * math.exp(-0.02)) < 4 * r.std_error
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_asian_mean_closed_form(ope, precision):
+    # E[sum_{i=1..N} S_{t_i} / N] = S0 mean_i e^{(r-q) t_i}, S0 excluded (mc.rs:88, 271)
+    n = 12
+    cfg = ope.McConfig(n_paths=400_000, n_steps=n, seed=5)
+    r = run(ope, precision, ope.Payoff.AsianArithmetic(0.0), cfg, hp=BASE, rate=0.03, q=0.0).result
+    want = math.exp(-0.03) * 100.0 * np.mean([math.exp(0.03 * (i + 1) / n) for i in range(n)])
+    assert abs(r.price - want) < 4 * r.std_error
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_barrier_at_or_below_spot_pays_discounted_rebate(ope, precision):
+    cfg = ope.McConfig(n_paths=10_000, n_steps=20, seed=1)
+    r = run(ope, precision, ope.Payoff.UpAndOut(100.0, 100.0, 3.0), cfg, hp=BASE, rate=0.05).result
+    # every path pays the rebate discounted from T (mc.rs:218, 276-277); only
+    # the FP64 summation rounding separates the mean from it
+    assert abs(r.price - math.exp(-0.05) * 3.0) < 1e-13
+    assert r.std_error < 1e-6
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_barrier_at_infinity_is_european_bitwise(ope, precision, scheme):
+    cfg = ope.McConfig(n_paths=50_000, n_steps=30, seed=9, scheme=scheme)
+    e = run(ope, precision, ope.Payoff.European(100.0), cfg, hp=BASE).result
+    u = run(ope, precision, ope.Payoff.UpAndOut(100.0, 1e300), cfg, hp=BASE).result
+    assert e == u
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_bates_zero_intensity_is_heston_bitwise(ope, precision):
+    cfg = ope.McConfig(n_paths=50_000, n_steps=30, seed=9)
+    h = run(ope, precision, ope.Payoff.European(100.0), cfg, hp=BASE).result
+    b = run(ope, precision, ope.Payoff.European(100.0), cfg, hp=BASE, jumps=(0.0, -0.1, 0.15)).result
+    assert h == b
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_deterministic_and_seed_dependent(ope, precision):
+    cfg = ope.McConfig(n_paths=2_200_000, n_steps=20, seed=11, scheme=1)   # 4 pairs per thread
+    a = run(ope, precision, ope.Payoff.AsianArithmetic(100.0), cfg, hp=BASE, jumps=(0.5, -0.1, 0.15))
+    b = run(ope, precision, ope.Payoff.AsianArithmetic(100.0), cfg, hp=BASE, jumps=(0.5, -0.1, 0.15))
+    assert a.result == b.result
+    cfg.seed = 12
+    c = run(ope, precision, ope.Payoff.AsianArithmetic(100.0), cfg, hp=BASE, jumps=(0.5, -0.1, 0.15))
+    assert c.result.price != a.result.price
+    assert abs(c.result.price - a.result.price) < 4 * math.hypot(a.result.std_error, c.result.std_error)
+
+
+def test_reference_api_shapes(ope):
+    # mc_heston / mc_bates with the reference's McConfig::default()
+    hp = ope.HestonParams(**REF)
+    cfg = ope.McConfig()
+    r1 = ope.mc_heston(100.0, 1.0, 0.05, 0.0, hp, ope.Payoff.AsianArithmetic(100.0), cfg)
+    r2 = ope.mc_bates(100.0, 1.0, 0.05, 0.0, hp, 0.5, -0.10, 0.15, ope.Payoff.UpAndOut(100.0, 130.0, 0.0), cfg)
+    assert 0 < r1.price < 10 and r1.std_error > 0
+    assert 0 < r2.price < 10 and r2.std_error > 0
