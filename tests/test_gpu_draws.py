@@ -51,6 +51,7 @@ def test_philox_draws_match_oracle(ope, ora, lam, precision):
         np.testing.assert_allclose(zb, ozb, rtol=0, atol=1e-14 * (1 + np.abs(ozb).max()))
         np.testing.assert_allclose(js, ojs, rtol=0, atol=1e-14)
     else:
-        np.testing.assert_allclose(za, oza, rtol=0, atol=5e-6 * (1 + np.abs(oza)))
-        np.testing.assert_allclose(zb, ozb, rtol=0, atol=5e-6 * (1 + np.abs(ozb)))
+        # hardware FP32 log/sqrt/sin/cos: a few ulp of the FP64 values
+        assert np.all(np.abs(za - oza) <= 5e-6 * (1 + np.abs(oza)))
+        assert np.all(np.abs(zb - ozb) <= 5e-6 * (1 + np.abs(ozb)))
         np.testing.assert_allclose(js, ojs, rtol=0, atol=1e-6)
