@@ -234,6 +234,8 @@ int make_desc(const Request& r, McDesc& d) {
         d.pois_thr0 = thr >= 4294967295.0 ? 0xFFFFFFFFu : (thr <= 0.0 ? 0u : (uint32_t)thr);
         d.mu_j = r.mu_j;
         d.sigma_j = r.sigma_j;
+        d.f_mu_j2 = (float)(kLog2eD * r.mu_j);
+        d.f_sigma_j2 = (float)(kLog2eD * r.sigma_j);
     }
     return OPE_OK;
 }

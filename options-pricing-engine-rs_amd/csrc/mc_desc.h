@@ -57,7 +57,8 @@ struct alignas(16) McDesc {
     float qe_s1, qe_s0;            // s2 = s1 v+ + s0
     float qe_q0, qe_q1, qe_q2;     // dy = q0 + q1 v+ + q2 v' + sqrt(max(q3 v+ + q4 v', 0)) zb
     float qe_q3, qe_q4;
-    float pad_f_[3];
+    float f_mu_j2, f_sigma_j2;     // log2e * mu_j, log2e * sigma_j
+    float pad_f_;
 };
 
 }  // namespace ope

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "mc_desc.h"
 
 namespace ope {
@@ -27,7 +29,13 @@ namespace {
 // the key schedule lives in scalar registers.
 constexpr uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
 constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
-constexpr uint32_t kTagHeston = 0u, kTagBates = 1u;
+
+// Substreams of one pair (the counter's fourth word):
+//   kTagNormal   call s/2: words (x,y) -> normals of step s for even s, (z,w) for odd s
+//   kTagPoisson  call s/4: word s%4 -> Poisson count of step s (Bates)
+//   kTagJump     call s:   words (x,y) -> the jump-size normal of step s, drawn
+//                          only when its Poisson count is non-zero
+constexpr uint32_t kTagNormal = 0u, kTagPoisson = 1u, kTagJump = 2u;
 
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -50,6 +58,9 @@ __device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf
 __device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float hw_sin_rev(float x) { return __builtin_amdgcn_sinf(x); }  // sin(2 pi x)
 __device__ __forceinline__ float hw_cos_rev(float x) { return __builtin_amdgcn_cosf(x); }  // cos(2 pi x)
+__device__ __forceinline__ float clamp_pos(float x) {                                     // max(x, 0), one v_med3
+    return __builtin_amdgcn_fmed3f(x, 0.0f, 3.402823466e38f);
+}
 
 constexpr double kLog2e = 1.4426950408889634;
 constexpr double kLn2 = 0.6931471805599453;
@@ -77,55 +88,10 @@ __device__ __forceinline__ void box_muller_f32(uint32_t w0, uint32_t w1, float& 
     zb = r * hw_sin_rev(u2);
 }
 
-// Inverse standard normal CDF, Wichura (1988) algorithm AS 241 (PPND16),
-// about 1e-16 relative.  Used only for jump sizes (a rare branch).
-__device__ __noinline__ double inv_normal_cdf(double p) {
-    const double q = p - 0.5;
-    if (fabs(q) <= 0.425) {
-        const double r = 0.180625 - q * q;
-        const double num = (((((((2.5090809287301226727e+3 * r + 3.3430575583588128105e+4) * r +
-                                 6.7265770927008700853e+4) * r + 4.5921953931549871457e+4) * r +
-                               1.3731693765509461125e+4) * r + 1.9715909503065514427e+3) * r +
-                             1.3314166789178437745e+2) * r + 3.3871328727963666080e+0);
-        const double den = (((((((5.2264952788528545610e+3 * r + 2.8729085735721942674e+4) * r +
-                                 3.9307895800092710610e+4) * r + 2.1213794301586595867e+4) * r +
-                               5.3941960214247511077e+3) * r + 6.8718700749205790830e+2) * r +
-                             4.2313330701600911252e+1) * r + 1.0);
-        return q * num / den;
-    }
-    double r = q < 0.0 ? p : 1.0 - p;
-    r = sqrt(-log(r));
-    double val;
-    if (r <= 5.0) {
-        r -= 1.6;
-        const double num = (((((((7.74545014278341407640e-4 * r + 2.27238449892691845833e-2) * r +
-                                 2.41780725177450611770e-1) * r + 1.27045825245236838258e+0) * r +
-                               3.64784832476320460504e+0) * r + 5.76949722146069140550e+0) * r +
-                             4.63033784615654529590e+0) * r + 1.42343711074968357734e+0);
-        const double den = (((((((1.05075007164441684324e-9 * r + 5.47593808499534494600e-4) * r +
-                                 1.51986665636164571966e-2) * r + 1.48103976427480074590e-1) * r +
-                               6.89767334985100004550e-1) * r + 1.67638483018380384940e+0) * r +
-                             2.05319162663775882187e+0) * r + 1.0);
-        val = num / den;
-    } else {
-        r -= 5.0;
-        const double num = (((((((2.01033439929228813265e-7 * r + 2.71155556874348757815e-5) * r +
-                                 1.24266094738807843860e-3) * r + 2.65321895265761230930e-2) * r +
-                               2.96560571828504891230e-1) * r + 1.78482653991729133580e+0) * r +
-                             5.46378491116411436990e+0) * r + 6.65790464350110377720e+0);
-        const double den = (((((((2.04426310338993978564e-15 * r + 1.42151175831644588870e-7) * r +
-                                 1.84631831751005468180e-5) * r + 7.86869131145613259100e-4) * r +
-                               1.48753612908506148525e-2) * r + 1.36929880922735805310e-1) * r +
-                             5.99832206555887937690e-1) * r + 1.0);
-        val = num / den;
-    }
-    return q < 0.0 ? -val : val;
-}
-
 // Exact Poisson count by inverse CDF of (w + 1/2) 2^-32.  The common n = 0
-// case is decided by one integer compare against a host-computed threshold;
-// the tail walks the CDF in FP64.
-__device__ __noinline__ uint32_t poisson_tail(uint32_t w, double mu, double e0) {
+// case is decided by one integer compare against a host-computed threshold
+// (w < thr0 <=> u < e^-mu); this tail walks the CDF in FP64 and returns >= 1.
+__device__ __noinline__ __attribute__((unused)) uint32_t poisson_tail(uint32_t w, double mu, double e0) {
     const double u = ((double)w + 0.5) * 0x1p-32;
     uint32_t k = 0;
     double p = e0, F = e0;
@@ -135,19 +101,6 @@ __device__ __noinline__ uint32_t poisson_tail(uint32_t w, double mu, double e0) 
         F = F + p;
     }
     return k;
-}
-
-// Bates jump of one pair-step from Philox words 2 (count) and 3 (size):
-// jump_sum = mu_j n + sigma_j sqrt(n) z (mc.rs:139-140), FP64 in both modes.
-__device__ __forceinline__ double bates_jump(const McDesc& d, uint32_t w2, uint32_t w3, uint32_t& n) {
-    n = 0;
-    if (w2 >= d.pois_thr0) n = poisson_tail(w2, d.pois_mu, d.pois_e0);
-    double j = 0.0;
-    if (n != 0) {
-        const double z = inv_normal_cdf(((double)w3 + 0.5) * 0x1p-32);
-        j = d.mu_j * (double)n + d.sigma_j * sqrt((double)n) * z;
-    }
-    return j;
 }
 
 // Philox substream of one pair: key = seed, counter = (call, pair lo, pair hi, tag).
@@ -165,6 +118,83 @@ __device__ __forceinline__ PairStream make_stream(const McDesc& d, uint64_t pair
     s.plo = (uint32_t)pair;
     s.phi = (uint32_t)(pair >> 32);
     return s;
+}
+
+// Jump of step s given its Poisson word: jump_sum = mu_j n + sigma_j sqrt(n) z
+// (mc.rs:139-140).  FP64 mode returns it in ln units; FP32 mode in log2 units.
+template <bool FP32>
+__device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& st, uint32_t s, uint32_t wp,
+                                             uint32_t& n) {
+    using R = typename std::conditional<FP32, float, double>::type;
+    R j = 0;
+    n = 0;
+    if (wp >= d.pois_thr0) {  // rare: P = 1 - e^{-lambda dt}
+        n = poisson_tail(wp, d.pois_mu, d.pois_e0);
+        const uint4 w = st.call(s, kTagJump);
+        if constexpr (FP32) {
+            float z, unused;
+            box_muller_f32(w.x, w.y, z, unused);
+            j = fmaf(d.f_sigma_j2 * hw_sqrt((float)n), z, d.f_mu_j2 * (float)n);
+        } else {
+            double z, unused;
+            box_muller_f64(w.x, w.y, z, unused);
+            j = d.mu_j * (double)n + d.sigma_j * sqrt((double)n) * z;
+        }
+    }
+    return j;
+}
+
+// Drives the Philox-mode draws of one pair through all steps, calling
+// step(za, zb, jump, n) once per step.  Heston: one Philox call per two
+// steps.  Bates: the same normals plus one Poisson call per four steps and,
+// only for a step whose count is non-zero, one jump call.
+template <bool FP32, bool JUMPS, class Step>
+__device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& st, Step&& step) {
+    using R = typename std::conditional<FP32, float, double>::type;
+    auto one = [&](uint32_t wa, uint32_t wb, uint32_t wp, uint32_t s) {
+        R za, zb;
+        if constexpr (FP32)
+            box_muller_f32(wa, wb, za, zb);
+        else
+            box_muller_f64(wa, wb, za, zb);
+        uint32_t n = 0;
+        R j = 0;
+        if constexpr (JUMPS) j = jump_of_step<FP32>(d, st, s, wp, n);
+        step(za, zb, j, n);
+    };
+    const uint32_t N = d.n_steps;
+    uint32_t i = 0;
+    if constexpr (JUMPS) {
+        for (; i + 4 <= N; i += 4) {
+            const uint4 p = st.call(i >> 2, kTagPoisson);
+            uint4 w = st.call(i >> 1, kTagNormal);
+            one(w.x, w.y, p.x, i);
+            one(w.z, w.w, p.y, i + 1);
+            w = st.call((i >> 1) + 1, kTagNormal);
+            one(w.x, w.y, p.z, i + 2);
+            one(w.z, w.w, p.w, i + 3);
+        }
+        if (i < N) {
+            const uint4 p = st.call(i >> 2, kTagPoisson);
+            uint4 w = st.call(i >> 1, kTagNormal);
+            one(w.x, w.y, p.x, i);
+            if (i + 1 < N) one(w.z, w.w, p.y, i + 1);
+            if (i + 2 < N) {
+                w = st.call((i >> 1) + 1, kTagNormal);
+                one(w.x, w.y, p.z, i + 2);
+            }
+        }
+    } else {
+        for (; i + 2 <= N; i += 2) {
+            const uint4 w = st.call(i >> 1, kTagNormal);
+            one(w.x, w.y, 0u, i);
+            one(w.z, w.w, 0u, i + 1);
+        }
+        if (i < N) {
+            const uint4 w = st.call(i >> 1, kTagNormal);
+            one(w.x, w.y, 0u, i);
+        }
+    }
 }
 
 // -------------------------------------------------- FP64 path (reference)
@@ -218,40 +248,66 @@ __device__ __forceinline__ void qe_f64(const McDesc& d, double& ln_s, double& v,
 // ------------------------------------------------------- FP32-path math
 // State y = log2(S/S0) kept in FP64 (one FP64 add per step removes the
 // accumulation drift of a long FP32 sum); increments, variance and all
-// transcendentals in FP32 on the hardware units.
+// transcendentals in FP32 on the hardware units.  Loop invariants are folded
+// into log2 units on the host.
+//
+// Euler for one sibling: sa = a2*z1 and sb = b2*z2 are per-pair products
+// (the antithetic sibling passes them negated).
 template <bool JUMPS>
-__device__ __forceinline__ void euler_f32(const McDesc& d, double& y, float& v, float z1, float z2,
-                                          float jl) {
-    const float vp = fmaxf(v, 0.0f);
+__device__ __forceinline__ void euler_f32(const McDesc& d, double& y, float& v, float sa, float sb, float jl) {
+    const float vp = clamp_pos(v);
     const float sv = hw_sqrt(vp);
-    float dy = fmaf(d.eu_a2 * sv, z1, fmaf(d.eu_a1, vp, d.eu_a0));
+    float dy = fmaf(sv, sa, fmaf(d.eu_a1, vp, d.eu_a0));
     if (JUMPS) dy += jl;
     y += (double)dy;
-    v = fmaxf(fmaf(d.eu_b2 * sv, z2, fmaf(d.eu_b1, vp, v + d.eu_b0)), 0.0f);
+    v = fmaxf(fmaf(sv, sb, fmaf(d.eu_b1, vp, v + d.eu_b0)), 0.0f);
 }
 
+// erfc(x) = t exp(-x^2 + P(t)), t = 1/(1 + x/2), x >= 0, with fractional
+// error below 1.2e-7 everywhere: the Chebyshev fit of Press, Teukolsky,
+// Vetterling and Flannery, Numerical Recipes (2nd ed.), section 6.2, erfcc.
+// Returns P(t) - x^2.
+__device__ __forceinline__ float erfc_exponent(float x, float t) {
+    float p = 0.17087277f;
+    p = fmaf(p, t, -0.82215223f);
+    p = fmaf(p, t, 1.48851587f);
+    p = fmaf(p, t, -1.13520398f);
+    p = fmaf(p, t, 0.27886807f);
+    p = fmaf(p, t, -0.18628806f);
+    p = fmaf(p, t, 0.09678418f);
+    p = fmaf(p, t, 0.37409196f);
+    p = fmaf(p, t, 1.00002368f);
+    p = fmaf(p, t, -1.26551223f);
+    return fmaf(-x, x, p);
+}
+
+// QE variance step in FP32.  The quadratic branch (psi <= 1.5) is computed
+// for every lane; the exponential branch only runs when a lane of the wave
+// needs it, and uses 1 - ncdf(z) = erfc(z/sqrt2)/2 directly, without the
+// cancellation of 1 - u.
 __device__ __forceinline__ float qe_variance_f32(const McDesc& d, float vp, float z) {
     const float m = fmaf(d.qe_m1, vp, d.qe_m0);
     const float s2 = fmaf(d.qe_s1, vp, d.qe_s0);
     const float psi = fmaxf(s2 * hw_rcp(fmaxf(m * m, 1e-36f)), 1e-12f);
-    if (psi <= 1.5f) {
-        const float t = 2.0f * hw_rcp(psi);
-        const float b2 = (t - 1.0f) + hw_sqrt(fmaxf(t * (t - 1.0f), 0.0f));
-        const float a = m * hw_rcp(1.0f + b2);
-        const float bz = hw_sqrt(b2) + z;
-        return a * (bz * bz);
+    const float t2 = 2.0f * hw_rcp(psi);
+    const float b2 = (t2 - 1.0f) + hw_sqrt(fmaxf(t2 * (t2 - 1.0f), 0.0f));
+    const float bz = hw_sqrt(b2) + z;
+    float vn = (m * hw_rcp(1.0f + b2)) * (bz * bz);
+    if (psi > 1.5f) {
+        const float q = 2.0f * hw_rcp(psi + 1.0f);  // 1 - p
+        const float x = fabsf(z) * 0.70710678118654752f;
+        const float t = hw_rcp(fmaf(0.5f, x, 1.0f));
+        const float e = 0.5f * t * hw_exp2(erfc_exponent(x, t) * 1.44269504088896341f);  // ncdf(-|z|)
+        const float phic = z >= 0.0f ? e : 1.0f - e;                                    // 1 - ncdf(z)
+        const float inv_beta = fmaxf(m, 1e-36f) * (0.5f * (psi + 1.0f));                // m / (1 - p)
+        vn = phic >= q ? 0.0f : (hw_log2(q * hw_rcp(phic)) * 0.69314718055994531f) * inv_beta;
     }
-    // exponential branch: 1 - ncdf(z) = erfc(z/sqrt2)/2 directly (no cancellation)
-    const float p = (psi - 1.0f) * hw_rcp(psi + 1.0f);
-    const float q = 1.0f - p;
-    const float one_m_u = 0.5f * erfcf(z * 0.70710678118654752f);
-    if (one_m_u >= q) return 0.0f;
-    return (hw_log2(q * hw_rcp(one_m_u)) * 0.69314718055994531f) * (fmaxf(m, 1e-36f) * hw_rcp(q));
+    return vn;
 }
 
 template <bool JUMPS>
 __device__ __forceinline__ void qe_f32(const McDesc& d, double& y, float& v, float za, float zb, float jl) {
-    const float vp = fmaxf(v, 0.0f);
+    const float vp = clamp_pos(v);
     const float vn = qe_variance_f32(d, vp, za);
     const float var = fmaxf(fmaf(d.qe_q3, vp, d.qe_q4 * vn), 0.0f);
     float dy = fmaf(hw_sqrt(var), zb, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, d.qe_q0)));
@@ -307,7 +363,7 @@ __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bo
     double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
     const uint32_t N = d.n_steps;
 
-    auto step = [&](double za, double zb, double jump) {
+    auto step = [&](double za, double zb, double jump, uint32_t) {
         if (SCHEME == kSchemeEuler) {
             const double z2 = d.rho * za + d.sqrt_1m_rho2 * zb;
             euler_f64<JUMPS>(d, la, va, za, z2, jump);
@@ -327,37 +383,12 @@ __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bo
     };
 
     if constexpr (Src::kPhilox) {
-        if (JUMPS) {
-            for (uint32_t i = 0; i < N; ++i) {
-                const uint4 w = src.st.call(i, kTagBates);
-                double za, zb;
-                box_muller_f64(w.x, w.y, za, zb);
-                uint32_t n;
-                const double j = bates_jump(d, w.z, w.w, n);
-                step(za, zb, j);
-            }
-        } else {
-            uint32_t i = 0;
-            for (; i + 1 < N; i += 2) {
-                const uint4 w = src.st.call(i >> 1, kTagHeston);
-                double za, zb;
-                box_muller_f64(w.x, w.y, za, zb);
-                step(za, zb, 0.0);
-                box_muller_f64(w.z, w.w, za, zb);
-                step(za, zb, 0.0);
-            }
-            if (i < N) {
-                const uint4 w = src.st.call(i >> 1, kTagHeston);
-                double za, zb;
-                box_muller_f64(w.x, w.y, za, zb);
-                step(za, zb, 0.0);
-            }
-        }
+        philox_steps<false, JUMPS>(d, src.st, step);
     } else {
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
             src.get(i, za, zb, j);
-            step(za, zb, j);
+            step(za, zb, j, 0u);
         }
     }
 
@@ -393,11 +424,12 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
     const uint32_t N = d.n_steps;
 
-    auto step = [&](float za, float zb, float jl) {
+    auto step = [&](float za, float zb, float jl, uint32_t) {
         if (SCHEME == kSchemeEuler) {
-            const float z2 = fmaf(d.f_rho, za, d.f_rhoc * zb);
-            euler_f32<JUMPS>(d, ya, va, za, z2, jl);
-            if (anti) euler_f32<JUMPS>(d, yb, vb, -za, -z2, jl);
+            const float sa = d.eu_a2 * za;
+            const float sb = d.eu_b2 * fmaf(d.f_rho, za, d.f_rhoc * zb);
+            euler_f32<JUMPS>(d, ya, va, sa, sb, jl);
+            if (anti) euler_f32<JUMPS>(d, yb, vb, -sa, -sb, jl);
         } else {
             qe_f32<JUMPS>(d, ya, va, za, zb, jl);
             if (anti) qe_f32<JUMPS>(d, yb, vb, -za, -zb, jl);
@@ -413,37 +445,12 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     };
 
     if constexpr (Src::kPhilox) {
-        if (JUMPS) {
-            for (uint32_t i = 0; i < N; ++i) {
-                const uint4 w = src.st.call(i, kTagBates);
-                float za, zb;
-                box_muller_f32(w.x, w.y, za, zb);
-                uint32_t n;
-                const double j = bates_jump(d, w.z, w.w, n);
-                step(za, zb, (float)(j * kLog2e));
-            }
-        } else {
-            uint32_t i = 0;
-            for (; i + 1 < N; i += 2) {
-                const uint4 w = src.st.call(i >> 1, kTagHeston);
-                float za, zb;
-                box_muller_f32(w.x, w.y, za, zb);
-                step(za, zb, 0.0f);
-                box_muller_f32(w.z, w.w, za, zb);
-                step(za, zb, 0.0f);
-            }
-            if (i < N) {
-                const uint4 w = src.st.call(i >> 1, kTagHeston);
-                float za, zb;
-                box_muller_f32(w.x, w.y, za, zb);
-                step(za, zb, 0.0f);
-            }
-        }
+        philox_steps<true, JUMPS>(d, src.st, step);
     } else {
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
             src.get(i, za, zb, j);
-            step((float)za, (float)zb, (float)(j * kLog2e));
+            step((float)za, (float)zb, (float)(j * kLog2e), 0u);
         }
     }
 

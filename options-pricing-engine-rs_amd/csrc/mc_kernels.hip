@@ -95,8 +95,9 @@ __global__ void k_debug_philox(uint32_t k0, uint32_t k1, const uint4* __restrict
     if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
 }
 
-// The draws of the pricing loops (sim_pair_f64 / sim_pair_f32), one thread per
-// pair, written [step][pair].
+// The draws of the pricing loops (philox_steps, as sim_pair_f64 / sim_pair_f32
+// consume them), one thread per pair, written [step][pair]; FP32 jumps are
+// converted back from log2 to ln units.
 __global__ void k_debug_draws(const McDesc* __restrict__ desc, uint64_t pair0, uint64_t n_pairs, int fp32,
                               double* __restrict__ za, double* __restrict__ zb, double* __restrict__ jump,
                               uint32_t* __restrict__ nj) {
@@ -104,35 +105,25 @@ __global__ void k_debug_draws(const McDesc* __restrict__ desc, uint64_t pair0, u
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_pairs) return;
     const PairStream st = make_stream(d, pair0 + i);
-    for (uint32_t s = 0; s < d.n_steps; ++s) {
-        uint4 w;
-        uint32_t wa, wb;
-        if (d.jumps) {
-            w = st.call(s, kTagBates);
-            wa = w.x;
-            wb = w.y;
-        } else {
-            w = st.call(s >> 1, kTagHeston);
-            wa = (s & 1) ? w.z : w.x;
-            wb = (s & 1) ? w.w : w.y;
-        }
-        double a, b;
-        if (fp32) {
-            float fa, fb;
-            box_muller_f32(wa, wb, fa, fb);
-            a = fa;
-            b = fb;
-        } else {
-            box_muller_f64(wa, wb, a, b);
-        }
-        uint32_t n = 0;
-        double j = 0.0;
-        if (d.jumps) j = bates_jump(d, w.z, w.w, n);
+    uint32_t s = 0;
+    auto rec = [&](auto a, auto b, auto j, uint32_t n) {
         const uint64_t o = (uint64_t)s * n_pairs + i;
-        za[o] = a;
-        zb[o] = b;
-        jump[o] = fp32 ? (double)(float)(j * kLog2e) * kLn2 : j;
+        za[o] = (double)a;
+        zb[o] = (double)b;
+        jump[o] = fp32 ? (double)j * kLn2 : (double)j;
         if (nj) nj[o] = n;
+        ++s;
+    };
+    if (fp32) {
+        if (d.jumps)
+            philox_steps<true, true>(d, st, rec);
+        else
+            philox_steps<true, false>(d, st, rec);
+    } else {
+        if (d.jumps)
+            philox_steps<false, true>(d, st, rec);
+        else
+            philox_steps<false, false>(d, st, rec);
     }
 }
 

@@ -203,7 +203,7 @@ def ncdf(x):
 M32 = 0xFFFFFFFF
 PHILOX_M0, PHILOX_M1 = 0xD2511F53, 0xCD9E8D57
 PHILOX_W0, PHILOX_W1 = 0x9E3779B9, 0xBB67AE85
-TAG_HESTON, TAG_BATES = 0, 1
+TAG_NORMAL, TAG_POISSON, TAG_JUMP = 0, 1, 2
 
 
 def philox4x32_10(c0, c1, c2, c3, k0, k1):
@@ -250,18 +250,24 @@ def poisson_from_word(w, mu, kmax=1024):
     return flat.reshape(np.shape(w))
 
 
+def _box_muller(wa, wb):
+    u1 = (wa.astype(np.float64) + 0.5) * 2.0 ** -32
+    u2 = wb.astype(np.float64) * 2.0 ** -32
+    r = np.sqrt(-2.0 * np.log(u1))
+    return r * np.cos(2.0 * np.pi * u2), r * np.sin(2.0 * np.pi * u2)
+
+
 def philox_draws(seed, pair_ids, n_steps, dt=None, lam=0.0, mu_j=0.0, sigma_j=0.0):
     """The GPU engine's Philox-mode draws, FP64, for the given pair indices.
 
-    Heston (lam == 0): step s uses Philox call s >> 1 with tag 0, words (0,1)
-    for even s and (2,3) for odd s, mapped by Box-Muller.  Bates (lam > 0):
-    step s uses call s with tag 1; words 0,1 -> Box-Muller normals, word 2 ->
-    Poisson count of mean lam*dt by inverse CDF, word 3 -> the jump normal by
-    the inverse normal CDF.  Counter = (call, pair lo, pair hi, tag), key =
-    (seed lo, seed hi).  Returns za, zb, jump_sum, n as [n_steps, n_pairs].
+    Counter = (call, pair lo, pair hi, tag), key = (seed lo, seed hi).
+    Normals of step s: call s >> 1 with tag 0, words (0,1) for even s and
+    (2,3) for odd s, mapped by Box-Muller.  Bates (lam > 0) adds the Poisson
+    count of step s, by inverse CDF of mean lam*dt, from word s % 4 of call
+    s >> 2 with tag 1, and, when that count is non-zero, the jump normal from
+    the Box-Muller cosine output of words (0,1) of call s with tag 2.
+    Returns za, zb, jump_sum, n as [n_steps, n_pairs].
     """
-    from scipy.special import ndtri
-
     pair_ids = np.asarray(pair_ids, dtype=np.uint64)
     npairs = pair_ids.size
     k0, k1 = int(seed) & M32, (int(seed) >> 32) & M32
@@ -272,28 +278,22 @@ def philox_draws(seed, pair_ids, n_steps, dt=None, lam=0.0, mu_j=0.0, sigma_j=0.
     js = np.zeros((n_steps, npairs))
     nn = np.zeros((n_steps, npairs), dtype=np.int64)
     bates = lam > 0.0
-    words_cache = {}
+
+    def call(c, tag):
+        return philox4x32_10(np.full(npairs, c, dtype=np.uint64), plo, phi, np.full(npairs, tag, dtype=np.uint64),
+                             k0, k1)
+
     for s in range(n_steps):
-        call = s if bates else s >> 1
-        if call not in words_cache:
-            words_cache.clear()
-            words_cache[call] = philox4x32_10(np.full(npairs, call, dtype=np.uint64), plo, phi,
-                                              np.full(npairs, TAG_BATES if bates else TAG_HESTON,
-                                                      dtype=np.uint64), k0, k1)
-        w = words_cache[call]
-        if bates or (s & 1) == 0:
-            wa, wb = w[0], w[1]
-        else:
-            wa, wb = w[2], w[3]
-        u1 = (wa.astype(np.float64) + 0.5) * 2.0 ** -32
-        u2 = wb.astype(np.float64) * 2.0 ** -32
-        r = np.sqrt(-2.0 * np.log(u1))
-        za[s] = r * np.cos(2.0 * np.pi * u2)
-        zb[s] = r * np.sin(2.0 * np.pi * u2)
+        w = call(s >> 1, TAG_NORMAL)
+        wa, wb = (w[0], w[1]) if (s & 1) == 0 else (w[2], w[3])
+        za[s], zb[s] = _box_muller(wa, wb)
         if bates:
             mu = lam * dt
-            n = poisson_from_word(w[2], mu)
+            p = call(s >> 2, TAG_POISSON)[s & 3]
+            n = poisson_from_word(p, mu)
             nn[s] = n
-            zj = ndtri((w[3].astype(np.float64) + 0.5) * 2.0 ** -32)
-            js[s] = np.where(n > 0, mu_j * n + sigma_j * np.sqrt(n) * zj, 0.0)
+            if np.any(n > 0):
+                wj = call(s, TAG_JUMP)
+                zj, _ = _box_muller(wj[0], wj[1])
+                js[s] = np.where(n > 0, mu_j * n + sigma_j * np.sqrt(n) * zj, 0.0)
     return za, zb, js, nn
