@@ -91,7 +91,7 @@ __device__ __forceinline__ void box_muller_f32(uint32_t w0, uint32_t w1, float& 
 // Exact Poisson count by inverse CDF of (w + 1/2) 2^-32.  The common n = 0
 // case is decided by one integer compare against a host-computed threshold
 // (w < thr0 <=> u < e^-mu); this tail walks the CDF in FP64 and returns >= 1.
-__device__ __noinline__ __attribute__((unused)) uint32_t poisson_tail(uint32_t w, double mu, double e0) {
+__device__ __forceinline__ uint32_t poisson_tail(uint32_t w, double mu, double e0) {
     const double u = ((double)w + 0.5) * 0x1p-32;
     uint32_t k = 0;
     double p = e0, F = e0;
@@ -145,7 +145,7 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
 }
 
 // Drives the Philox-mode draws of one pair through all steps, calling
-// step(za, zb, jump, n) once per step.  Heston: one Philox call per two
+// step(za, zb, jump, n, s) once per step s.  Heston: one Philox call per two
 // steps.  Bates: the same normals plus one Poisson call per four steps and,
 // only for a step whose count is non-zero, one jump call.
 template <bool FP32, bool JUMPS, class Step>
@@ -160,7 +160,7 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
         uint32_t n = 0;
         R j = 0;
         if constexpr (JUMPS) j = jump_of_step<FP32>(d, st, s, wp, n);
-        step(za, zb, j, n);
+        step(za, zb, j, n, s);
     };
     const uint32_t N = d.n_steps;
     uint32_t i = 0;
@@ -254,13 +254,13 @@ __device__ __forceinline__ void qe_f64(const McDesc& d, double& ln_s, double& v,
 // Euler for one sibling: sa = a2*z1 and sb = b2*z2 are per-pair products
 // (the antithetic sibling passes them negated).
 template <bool JUMPS>
-__device__ __forceinline__ void euler_f32(const McDesc& d, double& y, float& v, float sa, float sb, float jl) {
+__device__ __forceinline__ float euler_f32(const McDesc& d, float& v, float sa, float sb, float jl) {
     const float vp = clamp_pos(v);
     const float sv = hw_sqrt(vp);
     float dy = fmaf(sv, sa, fmaf(d.eu_a1, vp, d.eu_a0));
     if (JUMPS) dy += jl;
-    y += (double)dy;
     v = fmaxf(fmaf(sv, sb, fmaf(d.eu_b1, vp, v + d.eu_b0)), 0.0f);
+    return dy;
 }
 
 // erfc(x) = t exp(-x^2 + P(t)), t = 1/(1 + x/2), x >= 0, with fractional
@@ -306,14 +306,14 @@ __device__ __forceinline__ float qe_variance_f32(const McDesc& d, float vp, floa
 }
 
 template <bool JUMPS>
-__device__ __forceinline__ void qe_f32(const McDesc& d, double& y, float& v, float za, float zb, float jl) {
+__device__ __forceinline__ float qe_f32(const McDesc& d, float& v, float za, float zb, float jl) {
     const float vp = clamp_pos(v);
     const float vn = qe_variance_f32(d, vp, za);
     const float var = fmaxf(fmaf(d.qe_q3, vp, d.qe_q4 * vn), 0.0f);
     float dy = fmaf(hw_sqrt(var), zb, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, d.qe_q0)));
     if (JUMPS) dy += jl;
-    y += (double)dy;
     v = vn;
+    return dy;
 }
 
 // ------------------------------------------------------------ path state
@@ -363,7 +363,7 @@ __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bo
     double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
     const uint32_t N = d.n_steps;
 
-    auto step = [&](double za, double zb, double jump, uint32_t) {
+    auto step = [&](double za, double zb, double jump, uint32_t, uint32_t) {
         if (SCHEME == kSchemeEuler) {
             const double z2 = d.rho * za + d.sqrt_1m_rho2 * zb;
             euler_f64<JUMPS>(d, la, va, za, z2, jump);
@@ -388,7 +388,7 @@ __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bo
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
             src.get(i, za, zb, j);
-            step(za, zb, j, 0u);
+            step(za, zb, j, 0u, i);
         }
     }
 
@@ -424,15 +424,30 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
     const uint32_t N = d.n_steps;
 
-    auto step = [&](float za, float zb, float jl, uint32_t) {
+    // European payoffs need only y_T: increments are summed in FP32 over
+    // blocks of 4 steps and folded into the FP64 total once per block.  Asian
+    // and barrier payoffs read y every step and fold every increment.
+    float pa = 0.0f, pb = 0.0f;
+    auto accumulate = [&](double& y, float& part, float dy, uint32_t s) {
+        if (PAYOFF == kPayEuropean) {
+            part += dy;
+            if ((s & 3u) == 3u) {
+                y += (double)part;
+                part = 0.0f;
+            }
+        } else {
+            y += (double)dy;
+        }
+    };
+    auto step = [&](float za, float zb, float jl, uint32_t, uint32_t s) {
         if (SCHEME == kSchemeEuler) {
             const float sa = d.eu_a2 * za;
             const float sb = d.eu_b2 * fmaf(d.f_rho, za, d.f_rhoc * zb);
-            euler_f32<JUMPS>(d, ya, va, sa, sb, jl);
-            if (anti) euler_f32<JUMPS>(d, yb, vb, -sa, -sb, jl);
+            accumulate(ya, pa, euler_f32<JUMPS>(d, va, sa, sb, jl), s);
+            if (anti) accumulate(yb, pb, euler_f32<JUMPS>(d, vb, -sa, -sb, jl), s);
         } else {
-            qe_f32<JUMPS>(d, ya, va, za, zb, jl);
-            if (anti) qe_f32<JUMPS>(d, yb, vb, -za, -zb, jl);
+            accumulate(ya, pa, qe_f32<JUMPS>(d, va, za, zb, jl), s);
+            if (anti) accumulate(yb, pb, qe_f32<JUMPS>(d, vb, -za, -zb, jl), s);
         }
         if (PAYOFF == kPayAsian) {
             suma += hw_exp2((float)ya);
@@ -450,9 +465,11 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
             src.get(i, za, zb, j);
-            step((float)za, (float)zb, (float)(j * kLog2e), 0u);
+            step((float)za, (float)zb, (float)(j * kLog2e), 0u, i);
         }
     }
+    ya += (double)pa;  // the last partial block (European)
+    yb += (double)pb;
 
     const double nan = __builtin_nan("");
     auto finish = [&](PathState& e, double y, float v, float sum, double mx) {
@@ -504,27 +521,43 @@ __device__ __forceinline__ void price_tile(const McDesc& d, uint32_t tile, const
     const int tid = threadIdx.x;
     const bool anti = d.antithetic != 0;
     const uint64_t base = (uint64_t)tile * (uint64_t)(kBlock * PPT);
+    // A tile never straddles a multiple of 2^32 pairs (kBlock * PPT divides
+    // 2^32), so the high word of every pair index in it is the tile's: a
+    // uniform value, which lets the first Philox round run on the scalar unit.
+    const uint32_t pair_hi = (uint32_t)(base >> 32);
+    // ea/eb/valid are only ever indexed with compile-time constants so they
+    // stay in registers; the simulation loop over j is kept rolled (one copy
+    // of the step code) and hands its result over through selects.
     PathState ea[PPT], eb[PPT];
     bool valid[PPT];
-#pragma unroll
+#pragma nounroll
     for (int j = 0; j < PPT; ++j) {
         const uint64_t pair = base + (uint64_t)j * kBlock + (uint64_t)tid;
-        valid[j] = pair < d.n_pairs;
-        if (valid[j]) {
-            const auto src = make_src(d, pair);
+        const bool ok = pair < d.n_pairs;
+        PathState a, b;
+        if (ok) {
+            const auto src = make_src(d, pair, pair_hi);
             if constexpr (sizeof(Real) == 8)
-                sim_pair_f64<SCHEME, JUMPS, PAYOFF>(d, src, anti, ea[j], eb[j]);
+                sim_pair_f64<SCHEME, JUMPS, PAYOFF>(d, src, anti, a, b);
             else
-                sim_pair_f32<SCHEME, JUMPS, PAYOFF>(d, src, anti, ea[j], eb[j]);
+                sim_pair_f32<SCHEME, JUMPS, PAYOFF>(d, src, anti, a, b);
             if (sink.payoff) {
                 const double k0 = strikes[d.strike_off];
-                sink.payoff[pair * 2] = payoff_value(d, ea[j], k0);
-                sink.payoff[pair * 2 + 1] = anti ? payoff_value(d, eb[j], k0) : __builtin_nan("");
+                sink.payoff[pair * 2] = payoff_value(d, a, k0);
+                sink.payoff[pair * 2 + 1] = anti ? payoff_value(d, b, k0) : __builtin_nan("");
             }
             if (sink.state) {
                 double* o = sink.state + pair * 8;
-                o[0] = ea[j].ln_s; o[1] = ea[j].v; o[2] = ea[j].sum; o[3] = ea[j].max;
-                if (anti) { o[4] = eb[j].ln_s; o[5] = eb[j].v; o[6] = eb[j].sum; o[7] = eb[j].max; }
+                o[0] = a.ln_s; o[1] = a.v; o[2] = a.sum; o[3] = a.max;
+                if (anti) { o[4] = b.ln_s; o[5] = b.v; o[6] = b.sum; o[7] = b.max; }
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < PPT; ++jj) {
+            if (jj == j) {
+                ea[jj] = a;
+                eb[jj] = b;
+                valid[jj] = ok;
             }
         }
     }

@@ -10,9 +10,10 @@ namespace ope {
 namespace {
 
 struct PhiloxMaker {
-    __device__ __forceinline__ PhiloxSource operator()(const McDesc& d, uint64_t pair) const {
+    __device__ __forceinline__ PhiloxSource operator()(const McDesc& d, uint64_t pair, uint32_t pair_hi) const {
         PhiloxSource s;
         s.st = make_stream(d, pair);
+        s.st.phi = pair_hi;  // equal to pair >> 32, known uniform
         return s;
     }
 };
@@ -106,7 +107,7 @@ __global__ void k_debug_draws(const McDesc* __restrict__ desc, uint64_t pair0, u
     if (i >= n_pairs) return;
     const PairStream st = make_stream(d, pair0 + i);
     uint32_t s = 0;
-    auto rec = [&](auto a, auto b, auto j, uint32_t n) {
+    auto rec = [&](auto a, auto b, auto j, uint32_t n, uint32_t) {
         const uint64_t o = (uint64_t)s * n_pairs + i;
         za[o] = (double)a;
         zb[o] = (double)b;

@@ -13,7 +13,7 @@ struct ReplayMaker {
     const double* za;
     const double* zb;
     const double* jump;
-    __device__ __forceinline__ ReplaySource operator()(const McDesc& d, uint64_t pair) const {
+    __device__ __forceinline__ ReplaySource operator()(const McDesc& d, uint64_t pair, uint32_t) const {
         ReplaySource s;
         s.za = za;
         s.zb = zb;

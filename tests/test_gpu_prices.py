@@ -142,11 +142,17 @@ def test_barrier_at_or_below_spot_pays_discounted_rebate(ope, precision):
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("scheme", [0, 1])
-def test_barrier_at_infinity_is_european_bitwise(ope, precision, scheme):
+def test_barrier_at_infinity_is_european(ope, precision, scheme):
+    # pathwise identical payoffs: bit-identical in FP64; in FP32-path mode the
+    # European kernel sums log-price increments in 4-step FP32 blocks while the
+    # barrier kernel folds every step into FP64, so only rounding differs
     cfg = ope.McConfig(n_paths=50_000, n_steps=30, seed=9, scheme=scheme)
     e = run(ope, precision, ope.Payoff.European(100.0), cfg, hp=BASE).result
     u = run(ope, precision, ope.Payoff.UpAndOut(100.0, 1e300), cfg, hp=BASE).result
-    assert e == u
+    if precision == 0:
+        assert e == u
+    else:
+        assert abs(e.price - u.price) < 1e-3 * e.std_error
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
