@@ -141,12 +141,29 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary Bates-surface measurement")
     ap.add_argument("--profile", action="store_true", help="time FP32-path mode only (for rocprofv3 runs)")
+    ap.add_argument("--precision", type=int, default=1, choices=[0, 1],
+                    help="with --profile: 1 = FP32-path (default), 0 = FP64")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 1 GiB read + 1 GiB write elementwise kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path has no multi-GPU mode in this build "
               "(the design shards (request, pair-range) tiles with one 24-byte-per-strike reduction)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+
+        n = 1 << 28  # 2^28 float32 = 1 GiB read and 1 GiB written per launch
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.add(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                                               "launches": 3}}))
+        return
 
     import opemc as ope
     from opemc import work_model as wm
@@ -156,11 +173,13 @@ def main():
           file=sys.stderr, flush=True)
     desc, conf, run, path_steps, flops = make_workload(ope, wm, args.workload)
 
+    if args.profile:
+        p, se, ms, k = time_mode(run, args.precision, args.steps, args.warmup)
+        print(json.dumps({"workload": args.workload, "precision": args.precision, "ms_per_step": ms,
+                          "kernel_ms": k, "path_steps_per_step": path_steps}))
+        return
     p32, se32, ms32, k32 = time_mode(run, 1, args.steps, args.warmup)
     print(f"# fp32 path: {ms32:.3f} ms/step wall, {k32:.3f} ms kernel", file=sys.stderr, flush=True)
-    if args.profile:
-        print(json.dumps({"workload": args.workload, "fp32_ms_per_step": ms32, "fp32_kernel_ms": k32}))
-        return
     p64, se64, ms64, k64 = time_mode(run, 0, max(3, args.steps // 2), 1)
     print(f"# fp64: {ms64:.3f} ms/step wall, {k64:.3f} ms kernel", file=sys.stderr, flush=True)
 
