@@ -88,6 +88,65 @@ def make_workload(ope, wm, name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def draw_uneven_requests(ope, n, seed):
+    """Requests drawn as the uneven-batch configuration: v0, theta ~ U[0.01, 0.16],
+    kappa ~ U[0.5, 5], xi ~ U[0.1, 1], rho ~ U[-0.95, 0], lambda ~ U[0, 1]
+    (mu_j = -0.1, sigma_j = 0.15), payoff uniform over {European, Asian,
+    up-and-out B=130}, K=100, S0=100, T=1, r=0.03, Euler, paths log-uniform in
+    [1e4, 2e6] (even), steps uniform in [50, 500]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        v0, theta = rng.uniform(0.01, 0.16, 2)
+        hp = ope.HestonParams(v0, rng.uniform(0.5, 5.0), theta, rng.uniform(0.1, 1.0), rng.uniform(-0.95, 0.0))
+        kind = int(rng.integers(0, 3))
+        pay = [ope.Payoff.European(100.0), ope.Payoff.AsianArithmetic(100.0),
+               ope.Payoff.UpAndOut(100.0, 130.0, 0.0)][kind]
+        paths = max(2, int(np.exp(rng.uniform(np.log(1e4), np.log(2e6)))) // 2 * 2)
+        cfg = ope.McConfig(n_paths=paths, n_steps=int(rng.integers(50, 501)), seed=1000 + i)
+        reqs.append(ope.McRequest(100.0, 1.0, 0.03, 0.0, hp, pay, cfg, float(rng.uniform(0.0, 1.0)), -0.1, 0.15))
+    return reqs
+
+
+def serve(ope, precision, rate, seconds, target, seed=5):
+    """Poisson arrivals at `rate` req/s for `seconds`; each request runs until its
+    standard error <= target * price (cap: its drawn path count)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < seconds]
+    reqs = draw_uneven_requests(ope, len(arrivals), seed + 1)
+    with ope.Server(precision, max_requests=4096, memory_budget_bytes=64 << 30, idle_exit_ms=200.0) as srv:
+        warm = srv.submit(reqs[0], target)   # first launch outside the measured window
+        srv.wait(warm)
+        tickets = []
+        t0 = time.perf_counter()
+        for a, q in zip(arrivals, reqs):
+            while True:
+                dt = t0 + a - time.perf_counter()
+                if dt <= 0:
+                    break
+                time.sleep(dt if dt > 2e-4 else 0)
+            tickets.append(srv.submit(q, target))
+        got = [srv.wait(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([g.latency_ms for g in got])
+    ps = sum(g.path_steps for g in got)
+    return {"requests": len(got), "offered_rate_per_s": rate, "arrival_window_s": seconds,
+            "target_rel_se": target, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "path_steps_per_s": ps / wall, "wall_s": wall,
+            "mean_rounds": float(np.mean([g.rounds for g in got])),
+            "met_target": int(sum(1 for g in got if g.result.std_error <= target * abs(g.result.price))),
+            "peak_active": st["peak_active"], "peak_resident_bytes": st["peak_resident_bytes"],
+            "kernel_launches": st["kernel_launches"], "workgroups": st["workgroups"]}
+
+
 def time_mode(run, precision, steps, warmup):
     for _ in range(warmup):
         run(precision)
@@ -137,7 +196,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="qe_asian_uao", choices=["qe_asian_uao", "bates_surface", "heston_euler"])
+    ap.add_argument("--workload", default="qe_asian_uao",
+                    choices=["qe_asian_uao", "bates_surface", "heston_euler", "serving"])
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the serving measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary Bates-surface measurement")
     ap.add_argument("--profile", action="store_true", help="time FP32-path mode only (for rocprofv3 runs)")
@@ -171,6 +233,10 @@ def main():
     info = ope.device_info()
     print(f"# device {info['name']} {info['arch']} CUs={info['compute_units']} clock={info['clock_mhz']} MHz",
           file=sys.stderr, flush=True)
+    if args.workload == "serving":
+        out = {p: serve(ope, p, 2000.0, args.serve_seconds, 1e-3) for p in (1, 0)}
+        print(json.dumps({"serving": {"fp32": out[1], "fp64": out[0]}}))
+        return
     desc, conf, run, path_steps, flops = make_workload(ope, wm, args.workload)
 
     if args.profile:
@@ -213,7 +279,7 @@ def main():
         "prices_fp32": p32[:4],
         "prices_fp64": p64[:4],
         "std_errors": se64[:4],
-        "serving": None,
+        "serving": None,  # filled below
     }
     if args.workload == "qe_asian_uao" and not args.no_extra:
         d2, c2, run2, ps2, fl2 = make_workload(ope, wm, "bates_surface")
@@ -227,6 +293,15 @@ def main():
             "roofline_fp32": {"bound": "valu", "achieved": a2, "peak": wm.PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
                               "frac": a2 / wm.PEAK_TFLOPS["f32"]}}
         print(f"# bates surface fp32 {m32:.2f} ms fp64 {m64:.2f} ms", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        srv = {}
+        for p, name in ((1, "fp32"), (0, "fp64")):
+            srv[name] = serve(ope, p, 2000.0, args.serve_seconds, 1e-3)
+            print(f"# serving {name}: p50 {srv[name]['p50_ms']:.2f} ms p99 {srv[name]['p99_ms']:.2f} ms "
+                  f"{srv[name]['path_steps_per_s']:.3g} path-steps/s", file=sys.stderr, flush=True)
+        srv["workload"] = ("Poisson arrivals at 2000 req/s, requests drawn as the uneven-batch configuration, "
+                           "each run until std_error <= 1e-3 x price (cap: its drawn path count)")
+        line["serving"] = srv
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload)
     else:

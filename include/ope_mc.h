@@ -193,6 +193,65 @@ typedef struct ope_mc_request {
 int ope_mc_batch(const ope_mc_request *reqs, uint32_t n, const ope_mc_options *opts,
                  ope_mc_result *out);
 
+/* ---- serving: persistent-kernel request loop (no reference counterpart)
+ *
+ * Requests are admitted against a device-memory budget and a number of
+ * resident slots, in arrival order.  A persistent kernel keeps running the
+ * tiles of every admitted request; requests that arrive while others are on
+ * the GPU join as soon as a slot and budget free up, and their tiles run
+ * alongside the others'.  Each request's pair counter and partial sums stay
+ * resident on the device between rounds.  A request with target_rel_se > 0
+ * runs in rounds of 256-pair tiles until std_error <= target_rel_se * |price|
+ * or its n_paths are spent; the decision uses only the request's own sums, so
+ * a result does not depend on what else is being served.  A request that
+ * stops after P pairs returns bit-for-bit what ope_mc_bates_ex returns for
+ * n_paths = 2P (or P without antithetic pairs) when that call runs one pair
+ * per thread (fewer than 2^20 pairs). */
+typedef struct ope_server ope_server;
+
+typedef struct ope_server_config {
+    int32_t precision;            /* OPE_PRECISION_* for every request */
+    uint32_t max_requests;        /* resident request slots (0: 1024) */
+    uint64_t memory_budget_bytes; /* admission budget for resident request state (0: 64 GiB) */
+    uint32_t initial_round_pairs; /* pairs in a request's first round (0: 65536) */
+    double idle_exit_ms;          /* the persistent kernel exits after this long without work (0: 200) */
+} ope_server_config;
+
+typedef struct ope_server_request_info {
+    uint64_t paths;        /* paths simulated (both antithetic siblings count) */
+    uint64_t path_steps;
+    uint32_t rounds;
+    uint32_t slot;
+    double queue_ms;       /* submit -> admission */
+    double latency_ms;     /* submit -> result */
+} ope_server_request_info;
+
+typedef struct ope_server_stats {
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t path_steps;          /* simulated by completed requests */
+    uint64_t peak_resident_bytes; /* largest admitted footprint at one time */
+    uint32_t peak_active;         /* most requests resident at one time */
+    uint32_t kernel_launches;     /* persistent-kernel launches (relaunched after idle exits) */
+    uint32_t workgroups;          /* workgroups per launch (one of them polls the host) */
+    uint32_t pad_;
+} ope_server_stats;
+
+int ope_server_create(const ope_server_config *cfg, ope_server **out);
+/* Queues one request (validated here).  target_rel_se <= 0: run all n_paths. */
+int ope_server_submit(ope_server *s, const ope_mc_request *req, double target_rel_se, uint64_t *ticket);
+/* *done = 1 and the outputs filled once the request has finished. */
+int ope_server_poll(ope_server *s, uint64_t ticket, ope_mc_result *out, ope_server_request_info *info,
+                    int32_t *done);
+/* Blocks until the request finishes or timeout_ms passes (OPE_ERR_NOT_FOUND if
+ * it has not finished; negative timeout: no limit).  A finished ticket is
+ * forgotten after a successful wait. */
+int ope_server_wait(ope_server *s, uint64_t ticket, double timeout_ms, ope_mc_result *out,
+                    ope_server_request_info *info);
+int ope_server_get_stats(ope_server *s, ope_server_stats *out);
+/* Finishes the admitted requests, fails the queued ones, stops the kernel. */
+int ope_server_destroy(ope_server *s);
+
 /* ---- device / diagnostics */
 typedef struct ope_device_info {
     char name[64];

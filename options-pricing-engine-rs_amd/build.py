@@ -31,8 +31,10 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
 
 UNITS = [
     ("mc_kernels.hip", ["-ffp-contract=on"]),
+    ("mc_server.hip", ["-ffp-contract=on"]),
     ("mc_replay.hip", ["-ffp-contract=off"]),
     ("capi.cpp", []),
+    ("server.cpp", []),
 ]
 
 

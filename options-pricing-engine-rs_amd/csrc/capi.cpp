@@ -19,12 +19,12 @@
 #include <vector>
 
 #include "../../include/ope_mc.h"
+#include "engine.h"
 #include "mc_desc.h"
 #include "mc_launch.h"
 
-using namespace ope;
-
-namespace {
+namespace ope {
+namespace host {
 
 constexpr double kLog2eD = 1.4426950408889634;
 constexpr uint64_t kPPT4Threshold = 1ull << 20;  // pairs at which a request uses 4 pairs per thread
@@ -34,7 +34,6 @@ constexpr double kMaxPoissonMean = 64.0;
 
 thread_local std::string g_last_error;
 
-int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -44,14 +43,6 @@ int fail(int code, const char* fmt, ...) {
     g_last_error = buf;
     return code;
 }
-
-#define HIP_TRY(expr)                                                                                   \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return fail(e_ == hipErrorOutOfMemory ? OPE_ERR_OUT_OF_MEMORY : OPE_ERR_HIP, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                             \
-    } while (0)
 
 // Device buffer that only grows.
 struct DevBuf {
@@ -111,16 +102,6 @@ bool finite_all(std::initializer_list<double> xs) {
         if (!std::isfinite(x)) return false;
     return true;
 }
-
-// Everything one pricing simulation needs.
-struct Request {
-    double spot, expiry, rate, div_yield;
-    ope_heston_params hp;
-    bool has_jump_params;  // mc_bates (drift carries the jump compensator)
-    double lambda, mu_j, sigma_j;
-    ope_payoff payoff;
-    ope_mc_config cfg;
-};
 
 // Validates a request and fills its descriptor (everything except the
 // planning fields).  Expressions follow the reference's evaluation order.
@@ -524,7 +505,16 @@ Request make_request(double spot, double expiry, double rate, double div_yield, 
     return r;
 }
 
-}  // namespace
+int device_available() {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    return ensure_init(g_ctx);
+}
+
+}  // namespace host
+}  // namespace ope
+
+using namespace ope;
+using namespace ope::host;
 
 extern "C" {
 

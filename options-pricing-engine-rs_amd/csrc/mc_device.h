@@ -502,6 +502,25 @@ __device__ __forceinline__ void wave_sum2(double& s, double& q) {
     }
 }
 
+// Sum of n_tiles partial-sum slots slots[base + t * stride], t in tile order:
+// thread i takes tiles i, i + kBlock, ... sequentially, then the xor butterfly
+// over the wave, then waves 0..3 in order.  Every reduction of a request's
+// slots goes through this one function, so the order (and the bits) never
+// depend on which kernel reduces.  The result is valid in thread 0.
+__device__ __forceinline__ double2 block_sum_slots(const double2* slots, uint64_t base, uint32_t n_tiles,
+                                                   uint32_t stride, double2* w) {
+    double s = 0.0, q = 0.0;
+    for (uint32_t t = threadIdx.x; t < n_tiles; t += kBlock) {
+        const double2 v = slots[base + (uint64_t)t * stride];
+        s += v.x;
+        q += v.y;
+    }
+    wave_sum2(s, q);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = make_double2(s, q);
+    __syncthreads();
+    return make_double2(((w[0].x + w[1].x) + w[2].x) + w[3].x, ((w[0].y + w[1].y) + w[2].y) + w[3].y);
+}
+
 // Per-path outputs (replay mode only).
 struct PathSink {
     double* payoff;  // [pair][2]

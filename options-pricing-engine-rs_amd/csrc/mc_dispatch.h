@@ -1,0 +1,53 @@
+// Philox-mode tile bodies and their per-request dispatch, shared by the
+// batch kernels (mc_kernels.hip) and the persistent serving kernel
+// (mc_server.hip).  Both units are built with -ffp-contract=on, so a request
+// computes the same bits in either.
+#pragma once
+
+#include "mc_device.h"
+#include "mc_launch.h"
+
+namespace ope {
+namespace {
+
+struct PhiloxMaker {
+    __device__ __forceinline__ PhiloxSource operator()(const McDesc& d, uint64_t pair, uint32_t pair_hi) const {
+        PhiloxSource s;
+        s.st = make_stream(d, pair);
+        s.st.phi = pair_hi;  // equal to pair >> 32, known uniform
+        return s;
+    }
+};
+
+template <int CODE>
+struct Code {
+    static constexpr int kPPT = (CODE & 1) ? 4 : 1;
+    static constexpr int kPayoff = (CODE >> 1) % 3;
+    static constexpr bool kJumps = (((CODE >> 1) / 3) & 1) != 0;
+    static constexpr int kScheme = ((CODE >> 1) / 3) >> 1;
+};
+
+template <class Real, int CODE>
+__device__ __forceinline__ void run_code(const McDesc& d, uint32_t tile, const double* strikes, double2* slots,
+                                         double2* lds) {
+    using C = Code<CODE>;
+    price_tile<Real, C::kScheme, C::kJumps, C::kPayoff, C::kPPT>(d, tile, strikes, slots, PhiloxMaker{},
+                                                                 PathSink{nullptr, nullptr}, lds);
+}
+
+// Runs the body of config code `code` (uniform).  STEP = 2 from CODE = 0
+// instantiates only the one-pair-per-thread bodies.
+template <class Real, int CODE, int STEP = 1>
+__device__ __forceinline__ void dispatch(uint32_t code, const McDesc& d, uint32_t tile, const double* strikes,
+                                         double2* slots, double2* lds) {
+    if constexpr (CODE < (int)kNumCodes) {
+        if (code == (uint32_t)CODE) {
+            run_code<Real, CODE>(d, tile, strikes, slots, lds);
+            return;
+        }
+        dispatch<Real, CODE + STEP, STEP>(code, d, tile, strikes, slots, lds);
+    }
+}
+
+}  // namespace
+}  // namespace ope

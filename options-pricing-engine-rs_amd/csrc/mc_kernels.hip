@@ -4,35 +4,11 @@
 // in the mixed-batch kernel, and results do not depend on what shares a
 // launch.
 #include "mc_device.h"
+#include "mc_dispatch.h"
 #include "mc_launch.h"
 
 namespace ope {
 namespace {
-
-struct PhiloxMaker {
-    __device__ __forceinline__ PhiloxSource operator()(const McDesc& d, uint64_t pair, uint32_t pair_hi) const {
-        PhiloxSource s;
-        s.st = make_stream(d, pair);
-        s.st.phi = pair_hi;  // equal to pair >> 32, known uniform
-        return s;
-    }
-};
-
-template <int CODE>
-struct Code {
-    static constexpr int kPPT = (CODE & 1) ? 4 : 1;
-    static constexpr int kPayoff = (CODE >> 1) % 3;
-    static constexpr bool kJumps = (((CODE >> 1) / 3) & 1) != 0;
-    static constexpr int kScheme = ((CODE >> 1) / 3) >> 1;
-};
-
-template <class Real, int CODE>
-__device__ __forceinline__ void run_code(const McDesc& d, uint32_t tile, const double* strikes, double2* slots,
-                                         double2* lds) {
-    using C = Code<CODE>;
-    price_tile<Real, C::kScheme, C::kJumps, C::kPayoff, C::kPPT>(d, tile, strikes, slots, PhiloxMaker{},
-                                                                 PathSink{nullptr, nullptr}, lds);
-}
 
 // One block per tile; every request in the launch has config code CODE.
 template <class Real, int CODE>
@@ -43,18 +19,6 @@ __global__ __launch_bounds__(kBlock) void k_tiles_fixed(const McDesc* __restrict
     __shared__ double2 lds[kMaxStrikes * 4];
     const uint2 t = tiles[blockIdx.x];
     run_code<Real, CODE>(descs[t.x], t.y, strikes, slots, lds);
-}
-
-template <class Real, int CODE>
-__device__ __forceinline__ void dispatch(uint32_t code, const McDesc& d, uint32_t tile, const double* strikes,
-                                         double2* slots, double2* lds) {
-    if constexpr (CODE < (int)kNumCodes) {
-        if (code == (uint32_t)CODE) {
-            run_code<Real, CODE>(d, tile, strikes, slots, lds);
-            return;
-        }
-        dispatch<Real, CODE + 1>(code, d, tile, strikes, slots, lds);
-    }
 }
 
 // One block per tile; each tile's request picks its body (uniform branch).
@@ -76,18 +40,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const McDesc* __restrict__ de
     __shared__ double2 w[kBlock / 64];
     const uint2 jb = jobs[blockIdx.x];
     const McDesc& d = descs[jb.x];
-    const uint32_t nt = d.n_tiles, ns = d.n_strikes;
-    double s = 0.0, q = 0.0;
-    for (uint32_t t = threadIdx.x; t < nt; t += kBlock) {
-        const double2 v = slots[d.slot_off + (uint64_t)t * ns + jb.y];
-        s += v.x;
-        q += v.y;
-    }
-    wave_sum2(s, q);
-    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = make_double2(s, q);
-    __syncthreads();
-    if (threadIdx.x == 0)
-        out[d.out_off + jb.y] = make_double2(((w[0].x + w[1].x) + w[2].x) + w[3].x, ((w[0].y + w[1].y) + w[2].y) + w[3].y);
+    const double2 r = block_sum_slots(slots, d.slot_off + jb.y, d.n_tiles, d.n_strikes, w);
+    if (threadIdx.x == 0) out[d.out_off + jb.y] = r;
 }
 
 __global__ void k_debug_philox(uint32_t k0, uint32_t k1, const uint4* __restrict__ ctr, uint4* __restrict__ out,

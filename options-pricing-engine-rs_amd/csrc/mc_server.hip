@@ -1,0 +1,182 @@
+// Persistent serving kernel: workgroups loop over work items (one tile of one
+// resident request each) published by the host, until the host shuts the
+// server down or the queue stays empty for idle_limit polls.
+//
+// * Workgroup 0 is the sentinel (dispatched first, so always resident): it
+//   polls the host's tail and shutdown words (system scope) and mirrors the
+//   tail into device memory, so that the other workgroups poll L2 instead of
+//   the host bus.  It alone decides that the kernel drains: on host shutdown,
+//   or when neither the tail nor the claim head has moved for idle_limit
+//   polls.  Workers leave once draining is set and nothing is left to claim
+//   (or, as a safety bound, after 8 x idle_limit empty polls).
+// * A worker claims item h by a compare-and-swap on the device head while
+//   h < mirrored tail, reads the item and the request's record from host
+//   memory with system-scope loads (never through the scalar cache), copies
+//   the record into SGPRs, and runs the same tile body as the batch kernels.
+// * Completing a tile bumps done[slot]; the workgroup that brings it to the
+//   item's round target reduces the request's tiles in the canonical order
+//   (block_sum_slots, as k_reduce) and stores a completion record to host
+//   memory, its sequence word last, with release semantics.
+// Built with -ffp-contract=on, as the batch kernels.
+#include "mc_device.h"
+#include "mc_dispatch.h"
+#include "mc_server.h"
+
+namespace ope {
+namespace {
+
+constexpr int kRecWords = (int)(sizeof(SrvRecord) / 4);
+
+// Extra words after the mirrored control block in device memory.
+struct SrvMirror {
+    uint64_t tail;
+    uint32_t shutdown;
+    uint32_t pad_;
+};
+
+__device__ __forceinline__ uint64_t sys_load64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t sys_load32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ void sentinel(const SrvLaunch& L, SrvMirror* mirror) {
+    if (threadIdx.x != 0) return;
+    uint64_t last_t = ~0ull, last_h = ~0ull;
+    uint32_t idle = 0;
+    for (;;) {
+        const uint64_t t = sys_load64(&L.hc->tail);
+        const uint32_t stop = sys_load32(&L.hc->shutdown);
+        const uint64_t h = __hip_atomic_load(&L.dc->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t != last_t) __hip_atomic_store(&mirror->tail, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (t != last_t || h != last_h) {
+            last_t = t;
+            last_h = h;
+            idle = 0;
+        } else if (++idle > L.idle_limit) {
+            break;
+        }
+        if (stop) break;
+        __builtin_amdgcn_s_sleep(16);
+    }
+    __hip_atomic_store(&mirror->shutdown, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class Real>
+__global__ __launch_bounds__(kBlock) void k_server(SrvLaunch L, SrvMirror* mirror) {
+    __shared__ double2 lds[kMaxStrikes * 4];
+    __shared__ uint32_t s_rec[kRecWords];
+    __shared__ SrvItem s_item;
+    __shared__ uint32_t s_status;  // 0 work, 1 leave
+    __shared__ uint32_t s_last;
+    __shared__ double s_strike;
+    __shared__ double2 s_w[kBlock / 64];
+
+    if (blockIdx.x == 0) {
+        sentinel(L, mirror);
+        return;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) {
+            uint32_t status = 0, idle = 0;
+            SrvItem it{0, 0, 0, 0};
+            for (;;) {
+                const uint64_t t = __hip_atomic_load(&mirror->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t h = __hip_atomic_load(&L.dc->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (h < t) {
+                    if (__hip_atomic_compare_exchange_strong(&L.dc->head, &h, h + 1, __ATOMIC_ACQ_REL,
+                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        const SrvItem* src = L.ring + (h % L.ring_cap);
+                        it.slot = sys_load32(&src->slot);
+                        it.tile = sys_load32(&src->tile);
+                        it.target = sys_load32(&src->target);
+                        it.n_tiles = sys_load32(&src->n_tiles);
+                        break;
+                    }
+                    continue;  // another workgroup took item h
+                }
+                if (__hip_atomic_load(&mirror->shutdown, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+                    // draining: leave once the tail mirrored last is fully claimed
+                    const uint64_t t2 =
+                        __hip_atomic_load(&mirror->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    if (__hip_atomic_load(&L.dc->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= t2) {
+                        status = 1;
+                        break;
+                    }
+                    continue;
+                }
+                if (++idle > 8u * L.idle_limit) {
+                    status = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(32);
+            }
+            s_item = it;
+            s_status = status;
+        }
+        __syncthreads();
+        if (s_status) return;
+        const SrvItem it = s_item;
+
+        // the request's record: host memory -> LDS -> SGPRs
+        const uint32_t* rw = reinterpret_cast<const uint32_t*>(L.recs + it.slot);
+        for (int i = threadIdx.x; i < kRecWords; i += kBlock) s_rec[i] = sys_load32(rw + i);
+        __syncthreads();
+        SrvRecord rec;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&rec);
+#pragma unroll
+        for (int i = 0; i < kRecWords; ++i) dst[i] = __builtin_amdgcn_readfirstlane(s_rec[i]);
+        if (threadIdx.x == 0) s_strike = rec.strike;
+        __syncthreads();
+
+        dispatch<Real, 0, 2>(config_code(rec.d), rec.d, it.tile, &s_strike, L.slots, lds);
+
+        if (threadIdx.x == 0) {  // thread 0 wrote this tile's slot
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            const uint32_t prev =
+                __hip_atomic_fetch_add(L.done + it.slot, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = (prev + 1u == it.target) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (s_last) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const double2 r = block_sum_slots(L.slots, rec.d.slot_off, it.n_tiles, 1, s_w);
+            if (threadIdx.x == 0) {
+                const uint64_t idx =
+                    __hip_atomic_fetch_add(&L.dc->comp_head, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                SrvCompletion* c = L.comps + (idx % L.comp_cap);
+                __hip_atomic_store(&c->slot, it.slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(&c->target, it.target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(&c->n_tiles, it.n_tiles, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(reinterpret_cast<uint64_t*>(&c->sum), __double_as_longlong(r.x), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(reinterpret_cast<uint64_t*>(&c->sum_sq), __double_as_longlong(r.y),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(&c->seq, idx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t server_occupancy(int fp32, int* blocks_per_cu) {
+    return fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_server<float>, kBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_server<double>, kBlock, 0);
+}
+
+size_t server_mirror_bytes() { return sizeof(SrvMirror); }
+
+hipError_t launch_server(const SrvLaunch& L, void* mirror, hipStream_t s) {
+    if (L.n_blocks < 2) return hipErrorInvalidValue;
+    SrvMirror* m = static_cast<SrvMirror*>(mirror);
+    if (L.fp32)
+        hipLaunchKernelGGL(k_server<float>, dim3(L.n_blocks), dim3(kBlock), 0, s, L, m);
+    else
+        hipLaunchKernelGGL(k_server<double>, dim3(L.n_blocks), dim3(kBlock), 0, s, L, m);
+    return hipGetLastError();
+}
+
+}  // namespace ope

@@ -431,3 +431,108 @@ def debug_draws(seed, pair0, n_pairs, n_steps, expiry=1.0, lambda_=0.0, mu_j=0.0
                                  za.ctypes.data_as(P), zb.ctypes.data_as(P), js.ctypes.data_as(P),
                                  nj.ctypes.data_as(C.POINTER(C.c_uint32))))
     return za, zb, js, nj
+
+
+# ------------------------------------------------------------------ serving
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("max_requests", C.c_uint32), ("memory_budget_bytes", C.c_uint64),
+                ("initial_round_pairs", C.c_uint32), ("idle_exit_ms", C.c_double)]
+
+
+class _ServerRequestInfo(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("path_steps", C.c_uint64), ("rounds", C.c_uint32), ("slot", C.c_uint32),
+                ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("path_steps", C.c_uint64),
+                ("peak_resident_bytes", C.c_uint64), ("peak_active", C.c_uint32), ("kernel_launches", C.c_uint32),
+                ("workgroups", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+def _server_lib():
+    L = lib()
+    if not getattr(L, "_srv_ready", False):
+        L.ope_server_create.argtypes = [C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]
+        L.ope_server_submit.argtypes = [C.c_void_p, C.POINTER(_Request), C.c_double, C.POINTER(C.c_uint64)]
+        L.ope_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(_Result), C.POINTER(_ServerRequestInfo),
+                                      C.POINTER(C.c_int32)]
+        L.ope_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(_Result),
+                                      C.POINTER(_ServerRequestInfo)]
+        L.ope_server_get_stats.argtypes = [C.c_void_p, C.POINTER(_ServerStats)]
+        L.ope_server_destroy.argtypes = [C.c_void_p]
+        for n in ("ope_server_create", "ope_server_submit", "ope_server_poll", "ope_server_wait",
+                  "ope_server_get_stats", "ope_server_destroy"):
+            getattr(L, n).restype = C.c_int
+        L._srv_ready = True
+    return L
+
+
+@dataclasses.dataclass(frozen=True)
+class ServedResult:
+    result: McResult
+    paths: int
+    path_steps: int
+    rounds: int
+    slot: int
+    queue_ms: float
+    latency_ms: float
+
+
+class Server:
+    """Persistent-kernel request loop (ope_server_*).  submit() queues a
+    request (optionally with a target relative standard error), wait()/poll()
+    return its result; results do not depend on what else is being served."""
+
+    def __init__(self, precision=PRECISION_FP32_PATH, max_requests=1024, memory_budget_bytes=64 << 30,
+                 initial_round_pairs=65536, idle_exit_ms=200.0):
+        L = _server_lib()
+        cfg = _ServerConfig(precision, max_requests, memory_budget_bytes, initial_round_pairs, idle_exit_ms)
+        h = C.c_void_p()
+        _check(L.ope_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.precision = precision
+
+    def submit(self, req: McRequest, target_rel_se: float = 0.0) -> int:
+        t = C.c_uint64()
+        r = req._c()
+        _check(_server_lib().ope_server_submit(self._h, C.byref(r), target_rel_se, C.byref(t)))
+        return t.value
+
+    @staticmethod
+    def _served(res, info):
+        return ServedResult(McResult(res.price, res.std_error), info.paths, info.path_steps, info.rounds, info.slot,
+                            info.queue_ms, info.latency_ms)
+
+    def poll(self, ticket: int):
+        res, info, done = _Result(), _ServerRequestInfo(), C.c_int32()
+        _check(_server_lib().ope_server_poll(self._h, ticket, C.byref(res), C.byref(info), C.byref(done)))
+        return self._served(res, info) if done.value else None
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0) -> ServedResult:
+        res, info = _Result(), _ServerRequestInfo()
+        _check(_server_lib().ope_server_wait(self._h, ticket, timeout_ms, C.byref(res), C.byref(info)))
+        return self._served(res, info)
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(_server_lib().ope_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_ if k != "pad_"}
+
+    def close(self):
+        if self._h:
+            _server_lib().ope_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

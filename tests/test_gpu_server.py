@@ -1,0 +1,122 @@
+"""Persistent-kernel serving loop: results equal the batch path bit for bit,
+a request's result is the same alone and in a mix of concurrent requests
+(also when it stops at a target standard error), admission respects the
+memory budget, and the kernel exits when idle and is relaunched on demand."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+WAIT_MS = 120_000.0
+
+
+def uneven_requests(ope, n, seed, max_paths=400_000, min_steps=20, max_steps=120):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        v0, theta = rng.uniform(0.01, 0.16, 2)
+        hp = ope.HestonParams(v0, rng.uniform(0.5, 5.0), theta, rng.uniform(0.1, 1.0), rng.uniform(-0.95, 0.0))
+        kind = int(rng.integers(0, 3))
+        pay = [ope.Payoff.European(100.0), ope.Payoff.AsianArithmetic(100.0), ope.Payoff.UpAndOut(100.0, 130.0)][kind]
+        paths = int(np.exp(rng.uniform(np.log(1e4), np.log(max_paths)))) // 2 * 2
+        cfg = ope.McConfig(n_paths=paths, n_steps=int(rng.integers(min_steps, max_steps + 1)), seed=500 + i,
+                           scheme=int(rng.integers(0, 2)))
+        lam = float(rng.uniform(0.0, 1.0)) if rng.random() < 0.5 else 0.0
+        out.append(ope.McRequest(100.0, 1.0, 0.03, 0.0, hp, pay, cfg, lam, -0.1, 0.15))
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_full_runs_equal_the_batch_path(ope, precision):
+    reqs = uneven_requests(ope, 16, 3)
+    print(f"batch reference precision={precision}", flush=True)
+    batch, _, _ = ope.mc_batch(reqs, precision)
+    print("serving", flush=True)
+    with ope.Server(precision, idle_exit_ms=50.0) as srv:
+        tickets = [srv.submit(q) for q in reqs]
+        got = [srv.wait(t, WAIT_MS) for t in tickets]
+        st = srv.stats()
+    print("stats", st, flush=True)
+    for g, b, q in zip(got, batch, reqs):
+        assert g.result == b
+        assert g.paths == q.cfg.n_paths
+        assert g.path_steps == q.cfg.n_paths * q.cfg.n_steps
+        assert g.rounds == 1
+    assert st["completed"] == len(reqs)
+    assert st["kernel_launches"] >= 1
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_adaptive_result_is_the_same_alone_and_in_the_mix(ope, precision):
+    hp = ope.HestonParams(0.04, 1.5, 0.04, 0.5, -0.7)
+    probe = ope.McRequest(100.0, 1.0, 0.03, 0.0, hp, ope.Payoff.European(100.0),
+                          ope.McConfig(n_paths=1_800_000, n_steps=60, seed=77), 0.5, -0.1, 0.15)
+    target = 2e-3
+    print(f"probe alone precision={precision}", flush=True)
+    with ope.Server(precision, idle_exit_ms=50.0, initial_round_pairs=16384) as srv:
+        alone = srv.wait(srv.submit(probe, target), WAIT_MS)
+    print("probe in a mix", alone, flush=True)
+    others = uneven_requests(ope, 40, 9)
+    with ope.Server(precision, idle_exit_ms=50.0, initial_round_pairs=16384) as srv:
+        tickets = [srv.submit(q, 1e-3) for q in others[:20]]
+        t_probe = srv.submit(probe, target)
+        time.sleep(0.002)
+        tickets += [srv.submit(q, 1e-3) for q in others[20:]]
+        mixed = srv.wait(t_probe, WAIT_MS)
+        for t in tickets:
+            srv.wait(t, WAIT_MS)
+        st = srv.stats()
+    assert mixed.result == alone.result
+    assert mixed.paths == alone.paths and mixed.rounds == alone.rounds
+    assert alone.rounds >= 2
+    assert alone.result.std_error <= target * abs(alone.result.price) or alone.paths == probe.cfg.n_paths
+    assert st["peak_active"] > 1
+    # ... and equals the one-shot call with the path count the server stopped at
+    cfg = ope.McConfig(n_paths=alone.paths, n_steps=60, seed=77)
+    one = ope.mc_price_ex(100.0, 1.0, 0.03, 0.0, hp, ope.Payoff.European(100.0), cfg, (0.5, -0.1, 0.15),
+                          ope.McOptions(precision))
+    assert one.result == alone.result
+
+
+def test_budget_limits_residency(ope):
+    reqs = uneven_requests(ope, 10, 11)
+    for q in reqs:
+        q.cfg.n_paths = 50_000
+    # footprint per request: tiles x (16 B slot + 16 B ring item) + record + completion
+    one = ((50_000 // 2 + 255) // 256) * 32 + 1024
+    budget = int(2.5 * one)
+    with ope.Server(1, memory_budget_bytes=budget, idle_exit_ms=50.0) as srv:
+        tickets = [srv.submit(q) for q in reqs]
+        for t in tickets:
+            srv.wait(t, WAIT_MS)
+        st = srv.stats()
+    assert st["completed"] == len(reqs)
+    assert st["peak_resident_bytes"] <= budget
+    assert st["peak_active"] <= 2
+
+
+def test_idle_exit_and_relaunch(ope):
+    q = uneven_requests(ope, 1, 13)[0]
+    with ope.Server(1, idle_exit_ms=20.0) as srv:
+        a = srv.wait(srv.submit(q), WAIT_MS)
+        time.sleep(0.5)   # the kernel drains after 20 ms without work
+        b = srv.wait(srv.submit(q), WAIT_MS)
+        st = srv.stats()
+    assert a.result == b.result
+    assert st["kernel_launches"] >= 2
+
+
+def test_rejections(ope):
+    with ope.Server(1, memory_budget_bytes=4096, idle_exit_ms=20.0) as srv:
+        big = uneven_requests(ope, 1, 17)[0]
+        big.cfg.n_paths = 1_000_000
+        with pytest.raises(ope.OpeError) as e:
+            srv.submit(big)
+        assert e.value.status == ope.OPE_ERR_OVER_BUDGET
+        bad = uneven_requests(ope, 1, 17)[0]
+        bad.cfg.n_paths = 1
+        with pytest.raises(ope.OpeError) as e:
+            srv.submit(bad)
+        assert e.value.status == ope.OPE_ERR_INVALID_ARGUMENT
