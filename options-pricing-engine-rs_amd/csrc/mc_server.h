@@ -3,14 +3,13 @@
 //
 // Host -> device (pinned, fine-grained host memory; the device only loads):
 //   SrvHostCtrl.tail      number of work items published (release store by the host)
-//   SrvHostCtrl.shutdown  non-zero: workers leave once the queue is empty
+//   SrvHostCtrl.shutdown  non-zero: the kernel drains (published items still run)
 //   SrvItem ring[]        work item i at ring[i % ring_cap]
 //   SrvRecord recs[]      descriptor + strike of each resident request slot
 // Device -> host (pinned; the device only stores, the host only loads):
 //   SrvCompletion comps[] round-completion records; seq = index + 1 is stored
 //                         last with release semantics
 // Device only (device memory; device-scope atomics):
-//   SrvDevCtrl.head       work items claimed
 //   SrvDevCtrl.comp_head  completion records issued
 //   done[slot]            tiles finished for this slot, cumulative over every
 //                         request that ever used the slot (never reset)
@@ -54,8 +53,8 @@ struct alignas(16) SrvCompletion {
 };
 
 struct SrvDevCtrl {
-    uint64_t head;
     uint64_t comp_head;
+    uint64_t pad_;
 };
 
 struct SrvLaunch {
@@ -68,8 +67,8 @@ struct SrvLaunch {
     double2* slots;
     SrvCompletion* comps;
     uint32_t comp_cap;
-    uint32_t idle_limit;  // empty polls (each followed by a sleep) before a worker leaves
-    uint32_t n_blocks;
+    uint32_t idle_limit;  // polls without a new tail or ticket before the sentinel drains the kernel
+    uint32_t n_blocks;    // workgroup 0 is the sentinel, 1.. are workers
     int fp32;
 };
 
@@ -79,6 +78,7 @@ static_assert(sizeof(SrvItem) == 16, "work item layout");
 hipError_t launch_server(const SrvLaunch& L, void* mirror, hipStream_t s);
 hipError_t server_occupancy(int fp32, int* blocks_per_cu);
 size_t server_mirror_bytes();
+size_t server_mirror_launch_offset();  // bytes [offset, server_mirror_bytes()) are cleared before a launch
 static_assert(sizeof(SrvRecord) % 4 == 0, "record copied as 32-bit words");
 
 }  // namespace ope

@@ -6,13 +6,20 @@
 //   polls the host's tail and shutdown words (system scope) and mirrors the
 //   tail into device memory, so that the other workgroups poll L2 instead of
 //   the host bus.  It alone decides that the kernel drains: on host shutdown,
-//   or when neither the tail nor the claim head has moved for idle_limit
-//   polls.  Workers leave once draining is set and nothing is left to claim
-//   (or, as a safety bound, after 8 x idle_limit empty polls).
-// * A worker claims item h by a compare-and-swap on the device head while
-//   h < mirrored tail, reads the item and the request's record from host
-//   memory with system-scope loads (never through the scalar cache), copies
-//   the record into SGPRs, and runs the same tile body as the batch kernels.
+//   or when neither the tail nor the ticket counter has moved for idle_limit
+//   polls.  Before it sets the drain flag it records the last tail it
+//   mirrored as the next launch's base.
+// * A free worker takes the next ticket with one fetch-and-add (tickets are
+//   numbered from that base) and waits until the mirrored tail passes it: the
+//   ticket is the ring index of its next work item.  Idle workers hold the
+//   tickets just past the tail, so a new item starts at once on an idle
+//   workgroup, and the claim costs one uncontended atomic per item however
+//   many workgroups wait.  A worker leaves once draining is set and its
+//   ticket is at or beyond the final tail: every item below that tail was
+//   taken by a worker that runs it, and the rest wait for the next launch.
+// * A worker reads the item and the request's record from host memory with
+//   system-scope loads (never through the scalar cache), copies the record
+//   into SGPRs, and runs the same tile body as the batch kernels.
 // * Completing a tile bumps done[slot]; the workgroup that brings it to the
 //   item's round target reduces the request's tiles in the canonical order
 //   (block_sum_slots, as k_reduce) and stores a completion record to host
@@ -27,10 +34,14 @@ namespace {
 
 constexpr int kRecWords = (int)(sizeof(SrvRecord) / 4);
 
-// Extra words after the mirrored control block in device memory.
+// Device-memory words shared by the sentinel and the workers.
+// The host clears [tickets, end) before every launch; tail and next_base
+// carry over from the previous launch.
 struct SrvMirror {
-    uint64_t tail;
-    uint32_t shutdown;
+    uint64_t tail;       // host tail as last seen by the sentinel
+    uint64_t next_base;  // tail at which the last launch drained: tickets start there
+    uint64_t tickets;    // tickets taken in this launch
+    uint32_t shutdown;   // draining
     uint32_t pad_;
 };
 
@@ -43,23 +54,27 @@ __device__ __forceinline__ uint32_t sys_load32(const uint32_t* p) {
 
 __device__ void sentinel(const SrvLaunch& L, SrvMirror* mirror) {
     if (threadIdx.x != 0) return;
-    uint64_t last_t = ~0ull, last_h = ~0ull;
+    uint64_t last = ~0ull, last_tk = 0, mirrored = mirror->next_base;
     uint32_t idle = 0;
     for (;;) {
         const uint64_t t = sys_load64(&L.hc->tail);
         const uint32_t stop = sys_load32(&L.hc->shutdown);
-        const uint64_t h = __hip_atomic_load(&L.dc->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (t != last_t) __hip_atomic_store(&mirror->tail, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (t != last_t || h != last_h) {
-            last_t = t;
-            last_h = h;
+        const uint64_t tk = __hip_atomic_load(&mirror->tickets, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t != last) {
+            __hip_atomic_store(&mirror->tail, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            last = t;
+            mirrored = t;
+            idle = 0;
+        } else if (tk != last_tk) {
             idle = 0;
         } else if (++idle > L.idle_limit) {
             break;
         }
+        last_tk = tk;
         if (stop) break;
         __builtin_amdgcn_s_sleep(16);
     }
+    __hip_atomic_store(&mirror->next_base, mirrored, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&mirror->shutdown, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -77,38 +92,32 @@ __global__ __launch_bounds__(kBlock) void k_server(SrvLaunch L, SrvMirror* mirro
         sentinel(L, mirror);
         return;
     }
+    // written by the previous launch's sentinel, which has finished
+    const uint64_t base = __hip_atomic_load(&mirror->next_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         if (threadIdx.x == 0) {
-            uint32_t status = 0, idle = 0;
+            const uint64_t ticket =
+                base + __hip_atomic_fetch_add(&mirror->tickets, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t status = 0;
             SrvItem it{0, 0, 0, 0};
             for (;;) {
                 const uint64_t t = __hip_atomic_load(&mirror->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                uint64_t h = __hip_atomic_load(&L.dc->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (h < t) {
-                    if (__hip_atomic_compare_exchange_strong(&L.dc->head, &h, h + 1, __ATOMIC_ACQ_REL,
-                                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        const SrvItem* src = L.ring + (h % L.ring_cap);
-                        it.slot = sys_load32(&src->slot);
-                        it.tile = sys_load32(&src->tile);
-                        it.target = sys_load32(&src->target);
-                        it.n_tiles = sys_load32(&src->n_tiles);
-                        break;
-                    }
-                    continue;  // another workgroup took item h
+                if (ticket < t) {
+                    const uint64_t* src = reinterpret_cast<const uint64_t*>(L.ring + (ticket % L.ring_cap));
+                    const uint64_t w0 = sys_load64(src), w1 = sys_load64(src + 1);
+                    it.slot = (uint32_t)w0;
+                    it.tile = (uint32_t)(w0 >> 32);
+                    it.target = (uint32_t)w1;
+                    it.n_tiles = (uint32_t)(w1 >> 32);
+                    break;
                 }
                 if (__hip_atomic_load(&mirror->shutdown, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
-                    // draining: leave once the tail mirrored last is fully claimed
-                    const uint64_t t2 =
-                        __hip_atomic_load(&mirror->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    if (__hip_atomic_load(&L.dc->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= t2) {
+                    // the sentinel stored its final tail before the flag
+                    if (ticket >= __hip_atomic_load(&mirror->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                         status = 1;
                         break;
                     }
                     continue;
-                }
-                if (++idle > 8u * L.idle_limit) {
-                    status = 1;
-                    break;
                 }
                 __builtin_amdgcn_s_sleep(32);
             }
@@ -168,6 +177,7 @@ hipError_t server_occupancy(int fp32, int* blocks_per_cu) {
 }
 
 size_t server_mirror_bytes() { return sizeof(SrvMirror); }
+size_t server_mirror_launch_offset() { return offsetof(SrvMirror, tickets); }
 
 hipError_t launch_server(const SrvLaunch& L, void* mirror, hipStream_t s) {
     if (L.n_blocks < 2) return hipErrorInvalidValue;

@@ -227,8 +227,10 @@ void ope_server::release_gpu() {
 bool ope_server::kernel_running() { return hipStreamQuery(stream) == hipErrorNotReady; }
 
 int ope_server::launch() {
-    // a previous launch may have drained: clear the mirror (stream-ordered)
-    HIP_TRY(hipMemsetAsync(mirror, 0, server_mirror_bytes(), stream));
+    // clear this launch's ticket counter and drain flag (stream-ordered); the
+    // mirrored tail and the base the last launch drained at carry over
+    const size_t off = server_mirror_launch_offset();
+    HIP_TRY(hipMemsetAsync(static_cast<char*>(mirror) + off, 0, server_mirror_bytes() - off, stream));
     HIP_TRY(launch_server(L, mirror, stream));
     std::lock_guard<std::mutex> lk(mu);
     stats.kernel_launches++;
