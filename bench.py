@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Throughput of the Heston/Bates Monte Carlo path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload qe_asian_uao|bates_surface|heston_euler]
+    python bench.py [--steps K] [--warmup W]
+                    [--workload qe_asian_uao|bates_surface|heston_euler|uneven_batch|serving]
 
 One step is one pass of the hot path over one batch of synthetic input:
 
@@ -14,6 +15,13 @@ One step is one pass of the hot path over one batch of synthetic input:
   sigma_j=0.15; 4M paths x 252 steps per expiry, 8 expiries, 41 call strikes
   60..140 priced from shared paths.
 * heston_euler: the reference's CPU-sized case, 100k paths x 252 Euler steps.
+* uneven_batch: 256 independent Heston/Bates requests of uneven path and step
+  counts and mixed payoffs in one launch.
+* serving: the persistent-kernel server under Poisson arrivals at 2000 req/s,
+  each request run to a standard error of 1e-3 x price; p50/p99 latency.
+
+The default run also measures bates_surface, uneven_batch and serving and
+carries them in the same JSON line.
 
 The value is FP32-path mode path-steps/s (both antithetic siblings count);
 FP64 mode runs the same workload on identical Philox streams and the line
@@ -85,6 +93,22 @@ def make_workload(ope, wm, name):
         desc = "Heston European call, full-truncation Euler, 100k paths x 252 steps, seed 42"
         return (desc, dict(n_paths=100_000, n_steps=252, scheme="Euler"), run, 100_000 * 252,
                 wm.flops(100_000, 252, "euler", False))
+    if name == "uneven_batch":
+        reqs = draw_uneven_requests(ope, 256, 4)
+
+        def run(precision):
+            res, stats, dev_ms, k_ms = ope.mc_batch(reqs, precision, with_kernel_ms=True)
+            return [r.price for r in res], [r.std_error for r in res], k_ms
+
+        kinds = {0: "european", 1: "asian", 2: "up_and_out"}
+        path_steps = sum(q.cfg.n_paths * q.cfg.n_steps for q in reqs)
+        flops = sum(wm.flops(q.cfg.n_paths, q.cfg.n_steps, "euler", q.lambda_ > 0, 1, kinds[q.payoff.kind])
+                    for q in reqs)
+        desc = ("256 independent Heston/Bates requests in one launch, drawn as v0, theta ~ U[0.01, 0.16], "
+                "kappa ~ U[0.5, 5], xi ~ U[0.1, 1], rho ~ U[-0.95, 0], lambda ~ U[0, 1], payoff uniform over "
+                "{European, Asian, up-and-out}, paths log-uniform in [10k, 2M], steps uniform in [50, 500], Euler")
+        conf = dict(n_requests=256, total_paths=sum(q.cfg.n_paths for q in reqs), scheme="Euler", draw_seed=4)
+        return desc, conf, run, path_steps, flops
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -112,13 +136,18 @@ def draw_uneven_requests(ope, n, seed):
 
 def serve(ope, precision, rate, seconds, target, seed=5):
     """Poisson arrivals at `rate` req/s for `seconds`; each request runs until its
-    standard error <= target * price (cap: its drawn path count)."""
+    standard error <= target * price (cap: its drawn path count).  rate = inf:
+    2000 * seconds requests arrive at once, which measures the server's
+    saturated throughput."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16)
-    arrivals = np.cumsum(gaps)
-    arrivals = arrivals[arrivals < seconds]
+    if math.isinf(rate):
+        arrivals = np.zeros(int(2000.0 * seconds))
+    else:
+        gaps = rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16)
+        arrivals = np.cumsum(gaps)
+        arrivals = arrivals[arrivals < seconds]
     reqs = draw_uneven_requests(ope, len(arrivals), seed + 1)
     with ope.Server(precision, max_requests=4096, memory_budget_bytes=64 << 30, idle_exit_ms=200.0) as srv:
         warm = srv.submit(reqs[0], target)   # first launch outside the measured window
@@ -137,7 +166,7 @@ def serve(ope, precision, rate, seconds, target, seed=5):
         st = srv.stats()
     lat = np.array([g.latency_ms for g in got])
     ps = sum(g.path_steps for g in got)
-    return {"requests": len(got), "offered_rate_per_s": rate, "arrival_window_s": seconds,
+    return {"requests": len(got), "offered_rate_per_s": rate if not math.isinf(rate) else "burst", "arrival_window_s": seconds,
             "target_rel_se": target, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
             "path_steps_per_s": ps / wall, "wall_s": wall,
@@ -145,6 +174,41 @@ def serve(ope, precision, rate, seconds, target, seed=5):
             "met_target": int(sum(1 for g in got if g.result.std_error <= target * abs(g.result.price))),
             "peak_active": st["peak_active"], "peak_resident_bytes": st["peak_resident_bytes"],
             "kernel_launches": st["kernel_launches"], "workgroups": st["workgroups"]}
+
+
+def secondary(ope, wm, name):
+    """A second workload timed in the same run: FP32-path and FP64 throughput and error."""
+    d, c, run, ps, fl = make_workload(ope, wm, name)
+    q32, s32, m32, k32 = time_mode(run, 1, 3, 1)
+    q64, s64, m64, k64 = time_mode(run, 0, 2, 1)
+    err = [abs(a - b) / s for a, b, s in zip(q32, q64, s64) if s > 0]
+    a32 = fl / (k32 * 1e-3) / 1e12
+    a64 = fl / (k64 * 1e-3) / 1e12
+    print(f"# {name}: fp32 {m32:.2f} ms fp64 {m64:.2f} ms", file=sys.stderr, flush=True)
+    return {"workload": d, "config": c, "path_steps": ps,
+            "fp32_path_steps_per_s": ps / (m32 * 1e-3), "fp64_path_steps_per_s": ps / (m64 * 1e-3),
+            "fp32_ms": m32, "fp64_ms": m64, "fp32_kernel_ms": k32, "fp64_kernel_ms": k64,
+            "fp32_over_fp64": m64 / m32, "fp32_price_err_max_se": max(err),
+            "roofline_fp32": {"bound": "valu", "achieved": a32, "peak": wm.PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                              "frac": a32 / wm.PEAK_TFLOPS["f32"]},
+            "roofline_fp64": {"bound": "valu", "achieved": a64, "peak": wm.PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                              "frac": a64 / wm.PEAK_TFLOPS["f64"]}}
+
+
+def serving(ope, seconds):
+    """The serving configuration (Poisson arrivals at 2000 req/s) in both
+    precisions, plus the same requests all at once (saturated throughput)."""
+    srv = {}
+    for p, name in ((1, "fp32"), (0, "fp64")):
+        for rate, tag in ((2000.0, ""), (math.inf, "_burst")):
+            r = serve(ope, p, rate, seconds, 1e-3)
+            srv[name + tag] = r
+            print(f"# serving {name}{tag}: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms "
+                  f"{r['path_steps_per_s']:.3g} path-steps/s", file=sys.stderr, flush=True)
+    srv["workload"] = ("Poisson arrivals at 2000 req/s, requests drawn as the uneven-batch configuration, "
+                       "each run until std_error <= 1e-3 x price (cap: its drawn path count); _burst: the "
+                       "same number of requests submitted at once")
+    return srv
 
 
 def time_mode(run, precision, steps, warmup):
@@ -197,11 +261,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="qe_asian_uao",
-                    choices=["qe_asian_uao", "bates_surface", "heston_euler", "serving"])
+                    choices=["qe_asian_uao", "bates_surface", "heston_euler", "uneven_batch", "serving"])
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the serving measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the secondary Bates-surface measurement")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary Bates-surface and uneven-batch measurements")
     ap.add_argument("--profile", action="store_true", help="time FP32-path mode only (for rocprofv3 runs)")
     ap.add_argument("--precision", type=int, default=1, choices=[0, 1],
                     help="with --profile: 1 = FP32-path (default), 0 = FP64")
@@ -234,8 +299,7 @@ def main():
     print(f"# device {info['name']} {info['arch']} CUs={info['compute_units']} clock={info['clock_mhz']} MHz",
           file=sys.stderr, flush=True)
     if args.workload == "serving":
-        out = {p: serve(ope, p, 2000.0, args.serve_seconds, 1e-3) for p in (1, 0)}
-        print(json.dumps({"serving": {"fp32": out[1], "fp64": out[0]}}))
+        print(json.dumps({"serving": serving(ope, args.serve_seconds)}))
         return
     desc, conf, run, path_steps, flops = make_workload(ope, wm, args.workload)
 
@@ -282,26 +346,10 @@ def main():
         "serving": None,  # filled below
     }
     if args.workload == "qe_asian_uao" and not args.no_extra:
-        d2, c2, run2, ps2, fl2 = make_workload(ope, wm, "bates_surface")
-        q32, s32, m32, kk32 = time_mode(run2, 1, 3, 1)
-        q64, s64, m64, kk64 = time_mode(run2, 0, 2, 1)
-        e2 = [abs(a - b) / s for a, b, s in zip(q32, q64, s64) if s > 0]
-        a2 = fl2 / (kk32 * 1e-3) / 1e12
-        line["bates_surface"] = {
-            "workload": d2, "fp32_path_steps_per_s": ps2 / (m32 * 1e-3), "fp64_path_steps_per_s": ps2 / (m64 * 1e-3),
-            "fp32_ms": m32, "fp64_ms": m64, "fp32_over_fp64": m64 / m32, "fp32_price_err_max_se": max(e2),
-            "roofline_fp32": {"bound": "valu", "achieved": a2, "peak": wm.PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
-                              "frac": a2 / wm.PEAK_TFLOPS["f32"]}}
-        print(f"# bates surface fp32 {m32:.2f} ms fp64 {m64:.2f} ms", file=sys.stderr, flush=True)
+        for extra in ("bates_surface", "uneven_batch"):
+            line[extra] = secondary(ope, wm, extra)
     if not args.no_serve:
-        srv = {}
-        for p, name in ((1, "fp32"), (0, "fp64")):
-            srv[name] = serve(ope, p, 2000.0, args.serve_seconds, 1e-3)
-            print(f"# serving {name}: p50 {srv[name]['p50_ms']:.2f} ms p99 {srv[name]['p99_ms']:.2f} ms "
-                  f"{srv[name]['path_steps_per_s']:.3g} path-steps/s", file=sys.stderr, flush=True)
-        srv["workload"] = ("Poisson arrivals at 2000 req/s, requests drawn as the uneven-batch configuration, "
-                           "each run until std_error <= 1e-3 x price (cap: its drawn path count)")
-        line["serving"] = srv
+        line["serving"] = serving(ope, args.serve_seconds)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload)
     else:

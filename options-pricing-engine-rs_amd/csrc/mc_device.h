@@ -263,57 +263,92 @@ __device__ __forceinline__ float euler_f32(const McDesc& d, float& v, float sa, 
     return dy;
 }
 
-// erfc(x) = t exp(-x^2 + P(t)), t = 1/(1 + x/2), x >= 0, with fractional
-// error below 1.2e-7 everywhere: the Chebyshev fit of Press, Teukolsky,
-// Vetterling and Flannery, Numerical Recipes (2nd ed.), section 6.2, erfcc.
-// Returns P(t) - x^2.
-__device__ __forceinline__ float erfc_exponent(float x, float t) {
-    float p = 0.17087277f;
-    p = fmaf(p, t, -0.82215223f);
-    p = fmaf(p, t, 1.48851587f);
-    p = fmaf(p, t, -1.13520398f);
-    p = fmaf(p, t, 0.27886807f);
-    p = fmaf(p, t, -0.18628806f);
-    p = fmaf(p, t, 0.09678418f);
-    p = fmaf(p, t, 0.37409196f);
-    p = fmaf(p, t, 1.00002368f);
-    p = fmaf(p, t, -1.26551223f);
-    return fmaf(-x, x, p);
+// ncdf(-|z|) = erfc(x)/2, x = |z|/sqrt2, from erfc(x) = t exp(P(t) - x^2),
+// t = 1/(1 + x/2), which has fractional error below 1.2e-7 for every x >= 0:
+// the Chebyshev fit of Press, Teukolsky, Vetterling and Flannery, Numerical
+// Recipes (2nd ed.), section 6.2, erfcc.  Evaluated as t 2^(Q(t) - xs^2) with
+// Q = log2e P - 1 (the halving) and xs = x sqrt(log2e), so that no separate
+// scaling multiplies are needed.
+__device__ __forceinline__ float ncdf_neg_abs_f32(float z) {
+    constexpr double L = 1.4426950408889634;                               // log2 e
+    const float xs = fabsf(z) * 0.8493218002880191f;                       // |z| sqrt(log2e / 2)
+    const float t = hw_rcp(fmaf(xs, 0.41627730557884884f, 1.0f));          // 1/(1 + x/2)
+    float p = (float)(0.17087277 * L);
+    p = fmaf(p, t, (float)(-0.82215223 * L));
+    p = fmaf(p, t, (float)(1.48851587 * L));
+    p = fmaf(p, t, (float)(-1.13520398 * L));
+    p = fmaf(p, t, (float)(0.27886807 * L));
+    p = fmaf(p, t, (float)(-0.18628806 * L));
+    p = fmaf(p, t, (float)(0.09678418 * L));
+    p = fmaf(p, t, (float)(0.37409196 * L));
+    p = fmaf(p, t, (float)(1.00002368 * L));
+    p = fmaf(p, t, (float)(-1.26551223 * L - 1.0));
+    return t * hw_exp2(fmaf(-xs, xs, p));
 }
 
-// QE variance step in FP32.  The quadratic branch (psi <= 1.5) is computed
-// for every lane; the exponential branch only runs when a lane of the wave
-// needs it, and uses 1 - ncdf(z) = erfc(z/sqrt2)/2 directly, without the
-// cancellation of 1 - u.
-__device__ __forceinline__ float qe_variance_f32(const McDesc& d, float vp, float z) {
-    const float m = fmaf(d.qe_m1, vp, d.qe_m0);
-    const float s2 = fmaf(d.qe_s1, vp, d.qe_s0);
-    const float psi = fmaxf(s2 * hw_rcp(fmaxf(m * m, 1e-36f)), 1e-12f);
-    const float t2 = 2.0f * hw_rcp(psi);
-    const float b2 = (t2 - 1.0f) + hw_sqrt(fmaxf(t2 * (t2 - 1.0f), 0.0f));
+// QE variance step in FP32 (Andersen 2008, psi_c = 1.5), in terms of the
+// conditional mean m, mm2 = 2 m^2 and the conditional variance s2, with no
+// division by psi = s2/m^2:
+// * quadratic branch: 2/psi = mm2/s2, capped at 2e12 (the psi floor 1e-12),
+//   b^2 = 2/psi - 1 + sqrt(2/psi (2/psi - 1)), a = m/(1 + b^2),
+//   v' = a (b + z)^2 (meaningless when psi > 1.5, and then replaced);
+// * exponential branch, given phic = 1 - ncdf(z): 1 - p = mm2/(s2 + m^2),
+//   1/beta = m/(1 - p); v' = 0 when u <= p (phic >= 1 - p), else
+//   ln((1 - p)/phic)/beta, which avoids the cancellation of 1 - u.
+__device__ __forceinline__ float qe_quad_f32(float m, float mm2, float s2, float z) {
+    const float t2 = fminf(mm2 * hw_rcp(s2), 2e12f);
+    const float a = t2 - 1.0f;
+    const float b2 = a + hw_sqrt(t2 * a);
     const float bz = hw_sqrt(b2) + z;
-    float vn = (m * hw_rcp(1.0f + b2)) * (bz * bz);
-    if (psi > 1.5f) {
-        const float q = 2.0f * hw_rcp(psi + 1.0f);  // 1 - p
-        const float x = fabsf(z) * 0.70710678118654752f;
-        const float t = hw_rcp(fmaf(0.5f, x, 1.0f));
-        const float e = 0.5f * t * hw_exp2(erfc_exponent(x, t) * 1.44269504088896341f);  // ncdf(-|z|)
-        const float phic = z >= 0.0f ? e : 1.0f - e;                                    // 1 - ncdf(z)
-        const float inv_beta = fmaxf(m, 1e-36f) * (0.5f * (psi + 1.0f));                // m / (1 - p)
-        vn = phic >= q ? 0.0f : (hw_log2(q * hw_rcp(phic)) * 0.69314718055994531f) * inv_beta;
-    }
-    return vn;
+    return (m * hw_rcp(1.0f + b2)) * (bz * bz);
 }
 
+__device__ __forceinline__ float qe_expo_f32(float m, float mm2, float s2, float phic) {
+    const float q = mm2 * hw_rcp(fmaf(0.5f, mm2, s2));
+    const float v = (hw_log2(q * hw_rcp(phic)) * 0.69314718055994531f) * (m * hw_rcp(q));
+    return phic >= q ? 0.0f : v;
+}
+
+// One QE step of both siblings of a pair: sibling a draws (za, zb), sibling b
+// (-za, -zb).  |za| is common to both, so the exponential branch's erfc is
+// evaluated once per pair, and only when a lane of the wave needs it.  The
+// variances are >= 0 on entry (QE's v' is never negative, and v0 is clamped
+// by the caller).  Returns the log2-price increments.
 template <bool JUMPS>
-__device__ __forceinline__ float qe_f32(const McDesc& d, float& v, float za, float zb, float jl) {
-    const float vp = clamp_pos(v);
-    const float vn = qe_variance_f32(d, vp, za);
-    const float var = fmaxf(fmaf(d.qe_q3, vp, d.qe_q4 * vn), 0.0f);
-    float dy = fmaf(hw_sqrt(var), zb, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, d.qe_q0)));
-    if (JUMPS) dy += jl;
-    v = vn;
-    return dy;
+__device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& vb, bool anti, float za, float zb,
+                                            float jl, float& dya, float& dyb) {
+    const float ma = fmaf(d.qe_m1, va, d.qe_m0), sa = fmaf(d.qe_s1, va, d.qe_s0);
+    const float ma2 = 2.0f * (ma * ma);
+    float na = qe_quad_f32(ma, ma2, sa, za);
+    const bool xa = sa > 0.75f * ma2;  // psi > 1.5
+    float mb = 0.0f, sb = 0.0f, mb2 = 0.0f, nb = 0.0f;
+    bool xb = false;
+    if (anti) {
+        mb = fmaf(d.qe_m1, vb, d.qe_m0);
+        sb = fmaf(d.qe_s1, vb, d.qe_s0);
+        mb2 = 2.0f * (mb * mb);
+        nb = qe_quad_f32(mb, mb2, sb, -za);
+        xb = sb > 0.75f * mb2;
+    }
+    if (xa || xb) {
+        const float e = ncdf_neg_abs_f32(za);
+        if (xa) na = qe_expo_f32(ma, ma2, sa, za >= 0.0f ? e : 1.0f - e);  // 1 - ncdf(za)
+        if (xb) nb = qe_expo_f32(mb, mb2, sb, za <= 0.0f ? e : 1.0f - e);  // 1 - ncdf(-za)
+    }
+    auto incr = [&](float vp, float vn, float z) {
+        const float var = fmaxf(fmaf(d.qe_q3, vp, d.qe_q4 * vn), 0.0f);
+        float dy = fmaf(hw_sqrt(var), z, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, d.qe_q0)));
+        if (JUMPS) dy += jl;
+        return dy;
+    };
+    dya = incr(va, na, zb);
+    va = na;
+    if (anti) {
+        dyb = incr(vb, nb, -zb);
+        vb = nb;
+    } else {
+        dyb = 0.0f;
+    }
 }
 
 // ------------------------------------------------------------ path state
@@ -377,8 +412,11 @@ __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bo
             if (anti) sumb += exp(lb);
         }
         if (PAYOFF == kPayUpAndOut) {  // running max of S, tracked as max of ln S (exp is monotone)
+            // both unconditionally (mxb is unused without the sibling): a
+            // conditional second update is merged into one store through a
+            // selected address, which keeps the pair in scratch memory
             mxa = fmax(mxa, la);
-            if (anti) mxb = fmax(mxb, lb);
+            mxb = fmax(mxb, lb);
         }
     };
 
@@ -419,7 +457,8 @@ template <int SCHEME, bool JUMPS, int PAYOFF, class Src>
 __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bool anti, PathState& ea,
                                              PathState& eb) {
     double ya = 0.0, yb = 0.0;
-    float va = d.f_v0, vb = d.f_v0;
+    // QE only ever reads v+ = max(v, 0) and never produces a negative v
+    float va = SCHEME == kSchemeEuler ? d.f_v0 : clamp_pos(d.f_v0), vb = va;
     float suma = 0.0f, sumb = 0.0f;
     double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
     const uint32_t N = d.n_steps;
@@ -446,16 +485,20 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
             accumulate(ya, pa, euler_f32<JUMPS>(d, va, sa, sb, jl), s);
             if (anti) accumulate(yb, pb, euler_f32<JUMPS>(d, vb, -sa, -sb, jl), s);
         } else {
-            accumulate(ya, pa, qe_f32<JUMPS>(d, va, za, zb, jl), s);
-            if (anti) accumulate(yb, pb, qe_f32<JUMPS>(d, vb, -za, -zb, jl), s);
+            float dya, dyb;
+            qe_pair_f32<JUMPS>(d, va, vb, anti, za, zb, jl, dya, dyb);
+            accumulate(ya, pa, dya, s);
+            accumulate(yb, pb, dyb, s);  // dyb = 0 without the sibling
         }
+        // sibling b's running state is updated unconditionally (unused without
+        // the sibling): see sim_pair_f64
         if (PAYOFF == kPayAsian) {
             suma += hw_exp2((float)ya);
-            if (anti) sumb += hw_exp2((float)yb);
+            sumb += hw_exp2((float)yb);
         }
         if (PAYOFF == kPayUpAndOut) {
             mxa = fmax(mxa, ya);
-            if (anti) mxb = fmax(mxb, yb);
+            mxb = fmax(mxb, yb);
         }
     };
 
