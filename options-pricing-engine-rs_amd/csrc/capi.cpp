@@ -204,6 +204,7 @@ int make_desc(const Request& r, McDesc& d) {
     d.eu_a2 = (float)(kLog2eD * sqdt);
     d.eu_b0 = (float)(hp.kappa * hp.theta * dt);
     d.eu_b1 = (float)(-hp.kappa * dt);
+    d.eu_c1 = (float)(1.0 - hp.kappa * dt);
     d.eu_b2 = (float)(hp.sigma * sqdt);
     d.f_rho = (float)hp.rho;
     d.f_rhoc = (float)d.sqrt_1m_rho2;

@@ -52,13 +52,13 @@ struct alignas(16) McDesc {
     float f_v0;
     float eu_a0, eu_a1, eu_a2;     // dy = a0 + a1 v+ + a2 sqrt(v+) z1
     float eu_b0, eu_b1, eu_b2;     // v' = max(v + b0 + b1 v+ + b2 sqrt(v+) z2, 0)
+    float eu_c1;                   // 1 + b1 (v' for v >= 0: max(b0 + c1 v + b2 sqrt(v) z2, 0))
     float f_rho, f_rhoc;           // z2 = rho za + sqrt(1-rho^2) zb
     float qe_m1, qe_m0;            // m = m1 v+ + m0
     float qe_s1, qe_s0;            // s2 = s1 v+ + s0
     float qe_q0, qe_q1, qe_q2;     // dy = q0 + q1 v+ + q2 v' + sqrt(max(q3 v+ + q4 v', 0)) zb
     float qe_q3, qe_q4;
     float f_mu_j2, f_sigma_j2;     // log2e * mu_j, log2e * sigma_j
-    float pad_f_;
 };
 
 }  // namespace ope

@@ -148,9 +148,21 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
 // step(za, zb, jump, n, s) once per step s.  Heston: one Philox call per two
 // steps.  Bates: the same normals plus one Poisson call per four steps and,
 // only for a step whose count is non-zero, one jump call.
-template <bool FP32, bool JUMPS, class Step>
-__device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& st, Step&& step) {
+//
+// DEFER (FP32 European payoffs, whose log-price needs only the sum of the
+// jumps): step() gets jump 0, and a step with a non-zero count only records
+// (step, Poisson word) in one of two per-lane slots; the jumps of the slots
+// are drawn after the last step and added to *jsum.  A wave then runs the
+// rare jump code about twice per path instead of at every step where one of
+// its lanes jumps.  A lane's third and later events are drawn at once.  The
+// draws are the same words either way.
+template <bool FP32, bool JUMPS, bool DEFER = false, class Step>
+__device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& st, Step&& step,
+                                             float* jsum = nullptr) {
     using R = typename std::conditional<FP32, float, double>::type;
+    static_assert(!DEFER || (FP32 && JUMPS), "deferred jumps are an FP32 jump-model mode");
+    uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u;
+    float jd = 0.0f;
     auto one = [&](uint32_t wa, uint32_t wb, uint32_t wp, uint32_t s) {
         R za, zb;
         if constexpr (FP32)
@@ -159,7 +171,22 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
             box_muller_f64(wa, wb, za, zb);
         uint32_t n = 0;
         R j = 0;
-        if constexpr (JUMPS) j = jump_of_step<FP32>(d, st, s, wp, n);
+        if constexpr (DEFER) {
+            if (wp >= d.pois_thr0) {
+                if (es0 == ~0u) {
+                    es0 = s;
+                    ew0 = wp;
+                } else if (es1 == ~0u) {
+                    es1 = s;
+                    ew1 = wp;
+                } else {
+                    uint32_t nn;
+                    jd += jump_of_step<true>(d, st, s, wp, nn);
+                }
+            }
+        } else if constexpr (JUMPS) {
+            j = jump_of_step<FP32>(d, st, s, wp, n);
+        }
         step(za, zb, j, n, s);
     };
     const uint32_t N = d.n_steps;
@@ -194,6 +221,12 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
             const uint4 w = st.call(i >> 1, kTagNormal);
             one(w.x, w.y, 0u, i);
         }
+    }
+    if constexpr (DEFER) {
+        uint32_t nn;
+        if (es0 != ~0u) jd += jump_of_step<true>(d, st, es0, ew0, nn);
+        if (es1 != ~0u) jd += jump_of_step<true>(d, st, es1, ew1, nn);
+        *jsum += jd;
     }
 }
 
@@ -251,16 +284,18 @@ __device__ __forceinline__ void qe_f64(const McDesc& d, double& ln_s, double& v,
 // transcendentals in FP32 on the hardware units.  Loop invariants are folded
 // into log2 units on the host.
 //
-// Euler for one sibling: sa = a2*z1 and sb = b2*z2 are per-pair products
-// (the antithetic sibling passes them negated).
-template <bool JUMPS>
-__device__ __forceinline__ float euler_f32(const McDesc& d, float& v, float sa, float sb, float jl) {
-    const float vp = clamp_pos(v);
-    const float sv = hw_sqrt(vp);
-    float dy = fmaf(sv, sa, fmaf(d.eu_a1, vp, d.eu_a0));
-    if (JUMPS) dy += jl;
-    v = fmaxf(fmaf(sv, sb, fmaf(d.eu_b1, vp, v + d.eu_b0)), 0.0f);
-    return dy;
+// Euler for one sibling in FP32: sa = a2*z1 and sb = b2*z2 are per-pair
+// products (the antithetic sibling passes them negated).  v >= 0 on entry:
+// every step ends with max(., 0), and a negative v0 takes the first step in
+// sim_pair_f32 instead.  So v+ = v and v + b0 + b1 v+ = c1 v + b0 with
+// c1 = 1 + b1.  Returns acc + a1 v + sqrt(v) sa: the caller passes a0 (+ the
+// jump) for the increment itself, or a running partial sum to which the
+// constant a0 is added once per path.
+__device__ __forceinline__ float euler_f32(const McDesc& d, float& v, float sa, float sb, float acc) {
+    const float sv = hw_sqrt(v);
+    const float r = fmaf(sv, sa, fmaf(d.eu_a1, v, acc));
+    v = fmaxf(fmaf(sv, sb, fmaf(d.eu_c1, v, d.eu_b0)), 0.0f);
+    return r;
 }
 
 // ncdf(-|z|) = erfc(x)/2, x = |z|/sqrt2, from erfc(x) = t exp(P(t) - x^2),
@@ -464,9 +499,11 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     const uint32_t N = d.n_steps;
 
     // European payoffs need only y_T: increments are summed in FP32 over
-    // blocks of 4 steps and folded into the FP64 total once per block.  Asian
-    // and barrier payoffs read y every step and fold every increment.
-    float pa = 0.0f, pb = 0.0f;
+    // blocks of 4 steps and folded into the FP64 total once per block (Euler:
+    // without the constant a0, added as N a0 after the loop, and without the
+    // jumps, which both siblings share and which are summed once, in jsum).
+    // Asian and barrier payoffs read y every step and fold every increment.
+    float pa = 0.0f, pb = 0.0f, jsum = 0.0f;
     auto accumulate = [&](double& y, float& part, float dy, uint32_t s) {
         if (PAYOFF == kPayEuropean) {
             part += dy;
@@ -478,12 +515,37 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
             y += (double)dy;
         }
     };
+    const bool v0_neg = SCHEME == kSchemeEuler && d.f_v0 < 0.0f;
     auto step = [&](float za, float zb, float jl, uint32_t, uint32_t s) {
         if (SCHEME == kSchemeEuler) {
             const float sa = d.eu_a2 * za;
             const float sb = d.eu_b2 * fmaf(d.f_rho, za, d.f_rhoc * zb);
-            accumulate(ya, pa, euler_f32<JUMPS>(d, va, sa, sb, jl), s);
-            if (anti) accumulate(yb, pb, euler_f32<JUMPS>(d, vb, -sa, -sb, jl), s);
+            const bool first_neg = v0_neg && s == 0;  // v+ = 0: no diffusion, v' = max(v0 + b0, 0)
+            if (PAYOFF == kPayEuropean) {
+                if (JUMPS) jsum += jl;
+                if (first_neg) {
+                    va = vb = fmaxf(d.f_v0 + d.eu_b0, 0.0f);
+                } else {
+                    pa = euler_f32(d, va, sa, sb, pa);
+                    if (anti) pb = euler_f32(d, vb, -sa, -sb, pb);
+                }
+                if ((s & 3u) == 3u) {
+                    ya += (double)pa;
+                    yb += (double)pb;
+                    pa = pb = 0.0f;
+                }
+            } else {
+                const float c = JUMPS ? d.eu_a0 + jl : d.eu_a0;
+                float dya = c, dyb = c;
+                if (first_neg) {
+                    va = vb = fmaxf(d.f_v0 + d.eu_b0, 0.0f);
+                } else {
+                    dya = euler_f32(d, va, sa, sb, c);
+                    if (anti) dyb = euler_f32(d, vb, -sa, -sb, c);
+                }
+                ya += (double)dya;
+                yb += (double)dyb;
+            }
         } else {
             float dya, dyb;
             qe_pair_f32<JUMPS>(d, va, vb, anti, za, zb, jl, dya, dyb);
@@ -503,7 +565,8 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     };
 
     if constexpr (Src::kPhilox) {
-        philox_steps<true, JUMPS>(d, src.st, step);
+        constexpr bool kDefer = JUMPS && PAYOFF == kPayEuropean;
+        philox_steps<true, JUMPS, kDefer>(d, src.st, step, &jsum);
     } else {
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
@@ -511,8 +574,14 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
             step((float)za, (float)zb, (float)(j * kLog2e), 0u, i);
         }
     }
-    ya += (double)pa;  // the last partial block (European)
-    yb += (double)pb;
+    if (PAYOFF == kPayEuropean) {
+        // the last partial block, the Euler constant N a0 and the jumps
+        // (Euler: summed in the steps; QE: only those drawn after the loop)
+        double c = (double)jsum;
+        if (SCHEME == kSchemeEuler) c += (double)N * (double)d.eu_a0;
+        ya += (double)pa + c;
+        yb += (double)pb + c;
+    }
 
     const double nan = __builtin_nan("");
     auto finish = [&](PathState& e, double y, float v, float sum, double mx) {
