@@ -205,6 +205,13 @@ int make_desc(const Request& r, McDesc& d) {
     d.eu_b0 = (float)(hp.kappa * hp.theta * dt);
     d.eu_b1 = (float)(-hp.kappa * dt);
     d.eu_c1 = (float)(1.0 - hp.kappa * dt);
+    {
+        const double a2 = kLog2eD * sqdt;
+        d.eu_r2 = (float)(-2.0 * std::log(2.0) * a2 * a2);
+        d.eu_kb = (float)(hp.sigma / kLog2eD);  // b2 / a2
+        // |rho| > 1 gives NaN here as sqrt(1 - rho^2) does in the reference
+        d.eu_phi = (float)(std::acos(hp.rho) / 6.283185307179586);
+    }
     d.eu_b2 = (float)(hp.sigma * sqdt);
     d.f_rho = (float)hp.rho;
     d.f_rhoc = (float)d.sqrt_1m_rho2;

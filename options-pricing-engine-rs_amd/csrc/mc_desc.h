@@ -53,6 +53,7 @@ struct alignas(16) McDesc {
     float eu_a0, eu_a1, eu_a2;     // dy = a0 + a1 v+ + a2 sqrt(v+) z1
     float eu_b0, eu_b1, eu_b2;     // v' = max(v + b0 + b1 v+ + b2 sqrt(v+) z2, 0)
     float eu_c1;                   // 1 + b1 (v' for v >= 0: max(b0 + c1 v + b2 sqrt(v) z2, 0))
+    float eu_r2, eu_kb, eu_phi;    // -2 ln2 a2^2, b2/a2, acos(rho)/(2 pi): mixed Box-Muller
     float f_rho, f_rhoc;           // z2 = rho za + sqrt(1-rho^2) zb
     float qe_m1, qe_m0;            // m = m1 v+ + m0
     float qe_s1, qe_s0;            // s2 = s1 v+ + s0

@@ -156,19 +156,33 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
 // rare jump code about twice per path instead of at every step where one of
 // its lanes jumps.  A lane's third and later events are drawn at once.  The
 // draws are the same words either way.
-template <bool FP32, bool JUMPS, bool DEFER = false, class Step>
+//
+// MIX (FP32 Euler): step() gets the scaled, correlated normals of the Euler
+// step, sa = a2 za and sb = b2 (rho za + sqrt(1 - rho^2) zb), instead of
+// (za, zb).  With za = r cos t and zb = r sin t, rho = cos phi and
+// sqrt(1 - rho^2) = sin phi, the mixed normal is r cos(t - phi): one cosine
+// replaces the sine and the mixing products, and a2, b2 fold into r.
+template <bool FP32, bool JUMPS, bool DEFER = false, bool MIX = false, class Step>
 __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& st, Step&& step,
                                              float* jsum = nullptr) {
     using R = typename std::conditional<FP32, float, double>::type;
     static_assert(!DEFER || (FP32 && JUMPS), "deferred jumps are an FP32 jump-model mode");
+    static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
     uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u;
     float jd = 0.0f;
     auto one = [&](uint32_t wa, uint32_t wb, uint32_t wp, uint32_t s) {
         R za, zb;
-        if constexpr (FP32)
+        if constexpr (MIX) {
+            const float u1 = fmaf((float)wa, 0x1p-32f, 0x1p-33f);
+            const float wf = (float)wb;
+            const float ra = hw_sqrt(d.eu_r2 * hw_log2(u1));  // a2 sqrt(-2 ln u1)
+            za = ra * hw_cos_rev(wf * 0x1p-32f);
+            zb = (ra * d.eu_kb) * hw_cos_rev(fmaf(wf, 0x1p-32f, -d.eu_phi));
+        } else if constexpr (FP32) {
             box_muller_f32(wa, wb, za, zb);
-        else
+        } else {
             box_muller_f64(wa, wb, za, zb);
+        }
         uint32_t n = 0;
         R j = 0;
         if constexpr (DEFER) {
@@ -516,10 +530,14 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
         }
     };
     const bool v0_neg = SCHEME == kSchemeEuler && d.f_v0 < 0.0f;
+    constexpr bool kMixed = Src::kPhilox && SCHEME == kSchemeEuler;
     auto step = [&](float za, float zb, float jl, uint32_t, uint32_t s) {
         if (SCHEME == kSchemeEuler) {
-            const float sa = d.eu_a2 * za;
-            const float sb = d.eu_b2 * fmaf(d.f_rho, za, d.f_rhoc * zb);
+            // Philox mode passes sa, sb already mixed and scaled; sibling b is
+            // computed whether or not it exists (its state is then unused),
+            // which costs nothing more than selecting it out
+            const float sa = kMixed ? za : d.eu_a2 * za;
+            const float sb = kMixed ? zb : d.eu_b2 * fmaf(d.f_rho, za, d.f_rhoc * zb);
             const bool first_neg = v0_neg && s == 0;  // v+ = 0: no diffusion, v' = max(v0 + b0, 0)
             if (PAYOFF == kPayEuropean) {
                 if (JUMPS) jsum += jl;
@@ -527,7 +545,7 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
                     va = vb = fmaxf(d.f_v0 + d.eu_b0, 0.0f);
                 } else {
                     pa = euler_f32(d, va, sa, sb, pa);
-                    if (anti) pb = euler_f32(d, vb, -sa, -sb, pb);
+                    pb = euler_f32(d, vb, -sa, -sb, pb);
                 }
                 if ((s & 3u) == 3u) {
                     ya += (double)pa;
@@ -541,7 +559,7 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
                     va = vb = fmaxf(d.f_v0 + d.eu_b0, 0.0f);
                 } else {
                     dya = euler_f32(d, va, sa, sb, c);
-                    if (anti) dyb = euler_f32(d, vb, -sa, -sb, c);
+                    dyb = euler_f32(d, vb, -sa, -sb, c);
                 }
                 ya += (double)dya;
                 yb += (double)dyb;
@@ -566,7 +584,7 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
 
     if constexpr (Src::kPhilox) {
         constexpr bool kDefer = JUMPS && PAYOFF == kPayEuropean;
-        philox_steps<true, JUMPS, kDefer>(d, src.st, step, &jsum);
+        philox_steps<true, JUMPS, kDefer, kMixed>(d, src.st, step, &jsum);
     } else {
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
