@@ -1,0 +1,84 @@
+/*
+ * ope_cf.h -- analytic Heston / Bates European prices on the GPU by
+ * characteristic-function inversion: the reference's heston_price /
+ * bates_price and their chain batches, one wavefront per option.
+ *
+ * Drop-in replacements (same arguments and meaning; a status code and an
+ * output pointer instead of the bare f64):
+ *   ope_heston_price   <- heston::heston_price   (src/heston.rs:61-72)
+ *   ope_bates_price    <- bates::bates_price     (src/bates.rs:29-39)
+ *   ope_batch_heston   <- batch::batch_heston    (src/batch.rs:23-27)
+ *   ope_batch_bates    <- batch::batch_bates     (src/batch.rs:38-42)
+ * plus ope_cf_price_options, which takes a model per option (what a
+ * calibration's residual and Jacobian evaluations need).
+ *
+ * The quadrature is the reference's: Gil-Pelaez inversion of the Albrecher
+ * form CF (heston.rs:82-127, bates.rs:49-83), globally adaptive
+ * Gauss-Kronrod 15/7 on u = (1 - t)/t over [0, 1], tolerance 1e-8, at most
+ * 200 panels (heston.rs:180-229), with the same bisection decisions, so a
+ * price matches the reference's to the last bits of the FP64 special
+ * functions.  The reference's own behaviour is kept, including that the CF
+ * drift carries r only while the spot is discounted at q (heston.rs:126).
+ *
+ * Errors: OPE_ERR_INVALID_ARGUMENT for a null pointer, a non-finite
+ * argument, or spot / strike <= 0 (ln(S/K) undefined); OPE_ERR_NO_DEVICE
+ * without a GPU.  There is no CPU fallback.
+ */
+#ifndef OPE_CF_H
+#define OPE_CF_H
+
+#include "ope_mc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* types.rs:67-72 */
+typedef struct ope_bates_params {
+    ope_heston_params heston;
+    double lambda;  /* jumps per year */
+    double mu_j;    /* mean log-jump */
+    double sigma_j; /* log-jump volatility */
+} ope_bates_params;
+
+/* types.rs:17-25 (vol is the BSM volatility; the CF pricers ignore it) */
+typedef struct ope_option_contract {
+    double spot;
+    double strike;
+    double expiry;
+    double rate;
+    double div_yield;
+    double vol;
+    int32_t opt_type; /* OPE_CALL / OPE_PUT */
+    int32_t reserved_;
+} ope_option_contract;
+
+typedef struct ope_cf_option {
+    ope_option_contract contract;
+    ope_bates_params params; /* model == OPE_CF_HESTON: only params.heston is read */
+    int32_t model;           /* OPE_CF_HESTON or OPE_CF_BATES */
+    int32_t reserved_;
+} ope_cf_option;
+
+enum { OPE_CF_HESTON = 0, OPE_CF_BATES = 1 };
+
+/* heston.rs:61-72 */
+int ope_heston_price(double spot, double strike, double expiry, double rate, double div_yield,
+                     const ope_heston_params *params, int32_t opt_type, double *out);
+/* bates.rs:29-39 */
+int ope_bates_price(double spot, double strike, double expiry, double rate, double div_yield,
+                    const ope_bates_params *params, int32_t opt_type, double *out);
+/* batch.rs:23-27: out[i] = heston_price(contracts[i], params), one launch */
+int ope_batch_heston(const ope_option_contract *contracts, size_t n, const ope_heston_params *params, double *out);
+/* batch.rs:38-42 */
+int ope_batch_bates(const ope_option_contract *contracts, size_t n, const ope_bates_params *params, double *out);
+/* n options with their own models in one launch.  panels, when given,
+ * receives 2 entries per option (panels used by the P1 and P2 integrals);
+ * kernel_ms, when given, the GPU time of the launch. */
+int ope_cf_price_options(const ope_cf_option *options, size_t n, double *out, uint32_t *panels, double *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPE_CF_H */

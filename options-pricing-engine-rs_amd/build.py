@@ -5,8 +5,8 @@
 
 Every HIP translation unit is compiled for gfx950 with packed FP32 math
 disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).  Floating-point
-contraction is set per unit: the replay kernels are built unfused so that
-they repeat the reference's FP64 arithmetic exactly; the Philox kernels fuse
+contraction is set per unit: the replay kernels and the analytic pricer are
+built unfused so that they repeat the reference's FP64 arithmetic exactly; the Philox kernels fuse
 only within one source expression, so a request computes the same bits in
 every kernel that can run it.
 """
@@ -33,8 +33,10 @@ UNITS = [
     ("mc_kernels.hip", ["-ffp-contract=on"]),
     ("mc_server.hip", ["-ffp-contract=on"]),
     ("mc_replay.hip", ["-ffp-contract=off"]),
+    ("cf_kernels.hip", ["-ffp-contract=off"]),
     ("capi.cpp", []),
     ("server.cpp", []),
+    ("cf.cpp", []),
 ]
 
 

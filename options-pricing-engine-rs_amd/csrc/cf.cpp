@@ -1,0 +1,188 @@
+// C ABI of the analytic pricer (include/ope_cf.h): validation, descriptors,
+// one launch of k_cf (cf_kernels.hip) per call.  The single-option entry
+// points are batches of one.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "../../include/ope_cf.h"
+#include "cf_desc.h"
+#include "engine.h"
+
+namespace ope {
+hipError_t launch_cf(const CfDesc* opts, uint32_t n, double* out, CfStats* stats, hipStream_t s);
+}
+
+using namespace ope;
+using namespace ope::host;
+
+namespace {
+
+struct CfContext {
+    std::mutex mu;
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void* buf = nullptr;  // descriptors, then prices, then stats
+    size_t cap = 0;
+};
+
+CfContext g_cf;
+
+constexpr size_t kMaxOptions = 1u << 24;  // one launch: one workgroup per option
+
+int cf_init(CfContext& c) {
+    if (c.ready) return OPE_OK;
+    if (int rc = device_available()) return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&c.ev0));
+    HIP_TRY(hipEventCreate(&c.ev1));
+    c.ready = true;
+    return OPE_OK;
+}
+
+int make_cf_desc(const ope_option_contract& k, const ope_bates_params& bp, bool bates, CfDesc& d) {
+    const ope_heston_params& h = bp.heston;
+    for (double v : {k.spot, k.strike, k.expiry, k.rate, k.div_yield, h.v0, h.kappa, h.theta, h.sigma, h.rho})
+        if (!std::isfinite(v)) return fail(OPE_ERR_INVALID_ARGUMENT, "non-finite argument");
+    if (bates && !(std::isfinite(bp.lambda) && std::isfinite(bp.mu_j) && std::isfinite(bp.sigma_j)))
+        return fail(OPE_ERR_INVALID_ARGUMENT, "non-finite jump parameters");
+    if (!(k.spot > 0.0) || !(k.strike > 0.0))
+        return fail(OPE_ERR_INVALID_ARGUMENT, "spot and strike must be > 0 (got %g, %g)", k.spot, k.strike);
+    if (k.opt_type != OPE_CALL && k.opt_type != OPE_PUT)
+        return fail(OPE_ERR_INVALID_ARGUMENT, "unknown option type %d", k.opt_type);
+    d.spot = k.spot;
+    d.strike = k.strike;
+    d.expiry = k.expiry;
+    d.rate = k.rate;
+    d.div_yield = k.div_yield;
+    d.v0 = h.v0;
+    d.kappa = h.kappa;
+    d.theta = h.theta;
+    d.sigma = h.sigma;
+    d.rho = h.rho;
+    d.lambda = bates ? bp.lambda : 0.0;
+    d.mu_j = bates ? bp.mu_j : 0.0;
+    d.sigma_j = bates ? bp.sigma_j : 0.0;
+    d.bates = bates ? 1 : 0;
+    d.put = k.opt_type == OPE_PUT ? 1 : 0;
+    return OPE_OK;
+}
+
+int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, double* kernel_ms) {
+    const size_t n = descs.size();
+    if (n == 0) return OPE_OK;
+    if (n > kMaxOptions) return fail(OPE_ERR_INVALID_ARGUMENT, "more than %zu options in one call", kMaxOptions);
+    std::lock_guard<std::mutex> lk(g_cf.mu);
+    if (int rc = cf_init(g_cf)) return rc;
+    CfContext& c = g_cf;
+    const size_t b_desc = sizeof(CfDesc) * n, b_out = sizeof(double) * n, b_stats = sizeof(CfStats) * n;
+    const size_t need = b_desc + b_out + b_stats;
+    if (need > c.cap) {
+        if (c.buf) HIP_TRY(hipFree(c.buf));
+        c.buf = nullptr;
+        c.cap = 0;
+        HIP_TRY(hipMalloc(&c.buf, need));
+        c.cap = need;
+    }
+    char* base = static_cast<char*>(c.buf);
+    CfDesc* d_desc = reinterpret_cast<CfDesc*>(base);
+    double* d_out = reinterpret_cast<double*>(base + b_desc);
+    CfStats* d_stats = reinterpret_cast<CfStats*>(base + b_desc + b_out);
+    HIP_TRY(hipMemcpyAsync(d_desc, descs.data(), b_desc, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(hipEventRecord(c.ev0, c.stream));
+    HIP_TRY(launch_cf(d_desc, (uint32_t)n, d_out, panels ? d_stats : nullptr, c.stream));
+    HIP_TRY(hipEventRecord(c.ev1, c.stream));
+    HIP_TRY(hipMemcpyAsync(out, d_out, b_out, hipMemcpyDeviceToHost, c.stream));
+    std::vector<CfStats> st;
+    if (panels) {
+        st.resize(n);
+        HIP_TRY(hipMemcpyAsync(st.data(), d_stats, b_stats, hipMemcpyDeviceToHost, c.stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    if (panels)
+        for (size_t i = 0; i < n; ++i) {
+            panels[2 * i] = st[i].panels_p1;
+            panels[2 * i + 1] = st[i].panels_p2;
+        }
+    if (kernel_ms) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        *kernel_ms = ms;
+    }
+    return OPE_OK;
+}
+
+ope_option_contract contract(double spot, double strike, double expiry, double rate, double div_yield,
+                             int32_t opt_type) {
+    ope_option_contract k{};
+    k.spot = spot;
+    k.strike = strike;
+    k.expiry = expiry;
+    k.rate = rate;
+    k.div_yield = div_yield;
+    k.opt_type = opt_type;
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ope_heston_price(double spot, double strike, double expiry, double rate, double div_yield,
+                     const ope_heston_params* params, int32_t opt_type, double* out) {
+    if (!params || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    ope_bates_params bp{};
+    bp.heston = *params;
+    std::vector<CfDesc> d(1);
+    if (int rc = make_cf_desc(contract(spot, strike, expiry, rate, div_yield, opt_type), bp, false, d[0])) return rc;
+    return run_cf(d, out, nullptr, nullptr);
+}
+
+int ope_bates_price(double spot, double strike, double expiry, double rate, double div_yield,
+                    const ope_bates_params* params, int32_t opt_type, double* out) {
+    if (!params || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<CfDesc> d(1);
+    if (int rc = make_cf_desc(contract(spot, strike, expiry, rate, div_yield, opt_type), *params, true, d[0]))
+        return rc;
+    return run_cf(d, out, nullptr, nullptr);
+}
+
+int ope_batch_heston(const ope_option_contract* contracts, size_t n, const ope_heston_params* params, double* out) {
+    if (n == 0) return OPE_OK;
+    if (!contracts || !params || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    ope_bates_params bp{};
+    bp.heston = *params;
+    std::vector<CfDesc> d(n);
+    for (size_t i = 0; i < n; ++i)
+        if (int rc = make_cf_desc(contracts[i], bp, false, d[i])) return fail(rc, "option %zu: %s", i, g_last_error.c_str());
+    return run_cf(d, out, nullptr, nullptr);
+}
+
+int ope_batch_bates(const ope_option_contract* contracts, size_t n, const ope_bates_params* params, double* out) {
+    if (n == 0) return OPE_OK;
+    if (!contracts || !params || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<CfDesc> d(n);
+    for (size_t i = 0; i < n; ++i)
+        if (int rc = make_cf_desc(contracts[i], *params, true, d[i])) return fail(rc, "option %zu: %s", i, g_last_error.c_str());
+    return run_cf(d, out, nullptr, nullptr);
+}
+
+int ope_cf_price_options(const ope_cf_option* options, size_t n, double* out, uint32_t* panels, double* kernel_ms) {
+    if (kernel_ms) *kernel_ms = 0.0;
+    if (n == 0) return OPE_OK;
+    if (!options || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<CfDesc> d(n);
+    for (size_t i = 0; i < n; ++i) {
+        const ope_cf_option& o = options[i];
+        if (o.model != OPE_CF_HESTON && o.model != OPE_CF_BATES)
+            return fail(OPE_ERR_INVALID_ARGUMENT, "option %zu: unknown model %d", i, o.model);
+        if (int rc = make_cf_desc(o.contract, o.params, o.model == OPE_CF_BATES, d[i]))
+            return fail(rc, "option %zu: %s", i, g_last_error.c_str());
+    }
+    return run_cf(d, out, panels, kernel_ms);
+}
+
+}  // extern "C"

@@ -536,3 +536,8 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------- analytic pricers
+from .cf import (BatesParams, CfOption, OptionContract, batch_bates, batch_heston,  # noqa: E402
+                 bates_price, cf_price_options, heston_price)

@@ -1,0 +1,154 @@
+"""Python mirror of the reference's analytic pricers over the GPU kernel.
+
+``heston_price`` (heston.rs:61-72), ``bates_price`` (bates.rs:29-39),
+``batch_heston`` / ``batch_bates`` (batch.rs:23-27, 38-42), ``BatesParams``
+(types.rs:67-72) and ``OptionContract`` (types.rs:17-25), through the C ABI of
+``include/ope_cf.h``.  ``cf_price_options`` prices options with a model each
+(one launch) and can report the quadrature panels and the GPU time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import HestonParams, OptionType, _Heston, _check, lib
+
+CF_HESTON = 0
+CF_BATES = 1
+
+
+class _Bates(C.Structure):
+    _fields_ = [("heston", _Heston), ("lambda_", C.c_double), ("mu_j", C.c_double), ("sigma_j", C.c_double)]
+
+
+class _Contract(C.Structure):
+    _fields_ = [("spot", C.c_double), ("strike", C.c_double), ("expiry", C.c_double), ("rate", C.c_double),
+                ("div_yield", C.c_double), ("vol", C.c_double), ("opt_type", C.c_int32),
+                ("reserved_", C.c_int32)]
+
+
+class _CfOption(C.Structure):
+    _fields_ = [("contract", _Contract), ("params", _Bates), ("model", C.c_int32), ("reserved_", C.c_int32)]
+
+
+def _cf_lib():
+    L = lib()
+    if not getattr(L, "_cf_ready", False):
+        d = C.c_double
+        L.ope_heston_price.argtypes = [d, d, d, d, d, C.POINTER(_Heston), C.c_int32, C.POINTER(d)]
+        L.ope_bates_price.argtypes = [d, d, d, d, d, C.POINTER(_Bates), C.c_int32, C.POINTER(d)]
+        L.ope_batch_heston.argtypes = [C.POINTER(_Contract), C.c_size_t, C.POINTER(_Heston), C.POINTER(d)]
+        L.ope_batch_bates.argtypes = [C.POINTER(_Contract), C.c_size_t, C.POINTER(_Bates), C.POINTER(d)]
+        L.ope_cf_price_options.argtypes = [C.POINTER(_CfOption), C.c_size_t, C.POINTER(d),
+                                           C.POINTER(C.c_uint32), C.POINTER(d)]
+        for n in ("ope_heston_price", "ope_bates_price", "ope_batch_heston", "ope_batch_bates",
+                  "ope_cf_price_options"):
+            getattr(L, n).restype = C.c_int
+        L._cf_ready = True
+    return L
+
+
+@dataclasses.dataclass(frozen=True)
+class BatesParams:
+    """types.rs:67-72"""
+    heston: HestonParams
+    lambda_: float
+    mu_j: float
+    sigma_j: float
+
+    def _c(self):
+        return _Bates(self.heston._c(), self.lambda_, self.mu_j, self.sigma_j)
+
+
+@dataclasses.dataclass(frozen=True)
+class OptionContract:
+    """types.rs:17-25 (vol is the BSM volatility; the CF pricers ignore it)."""
+    spot: float
+    strike: float
+    expiry: float
+    rate: float
+    div_yield: float
+    vol: float = 0.0
+    opt_type: OptionType = OptionType.Call
+
+    def _c(self):
+        return _Contract(self.spot, self.strike, self.expiry, self.rate, self.div_yield, self.vol,
+                         int(self.opt_type), 0)
+
+
+def heston_price(spot, strike, expiry, rate, div_yield, params: HestonParams,
+                 opt_type: OptionType = OptionType.Call) -> float:
+    out = C.c_double()
+    hp = params._c()
+    _check(_cf_lib().ope_heston_price(spot, strike, expiry, rate, div_yield, C.byref(hp), int(opt_type),
+                                      C.byref(out)))
+    return out.value
+
+
+def bates_price(spot, strike, expiry, rate, div_yield, params: BatesParams,
+                opt_type: OptionType = OptionType.Call) -> float:
+    out = C.c_double()
+    bp = params._c()
+    _check(_cf_lib().ope_bates_price(spot, strike, expiry, rate, div_yield, C.byref(bp), int(opt_type),
+                                     C.byref(out)))
+    return out.value
+
+
+def _contracts(contracts: Sequence[OptionContract]):
+    arr = (_Contract * max(1, len(contracts)))()
+    for i, c in enumerate(contracts):
+        arr[i] = c._c()
+    return arr
+
+
+def batch_heston(contracts: Sequence[OptionContract], params: HestonParams) -> np.ndarray:
+    out = np.zeros(len(contracts))
+    hp = params._c()
+    _check(_cf_lib().ope_batch_heston(_contracts(contracts), len(contracts), C.byref(hp),
+                                      out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
+def batch_bates(contracts: Sequence[OptionContract], params: BatesParams) -> np.ndarray:
+    out = np.zeros(len(contracts))
+    bp = params._c()
+    _check(_cf_lib().ope_batch_bates(_contracts(contracts), len(contracts), C.byref(bp),
+                                     out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
+@dataclasses.dataclass(frozen=True)
+class CfOption:
+    """One option with its own model: params is HestonParams or BatesParams."""
+    contract: OptionContract
+    params: object
+
+    def _c(self):
+        if isinstance(self.params, BatesParams):
+            return _CfOption(self.contract._c(), self.params._c(), CF_BATES, 0)
+        return _CfOption(self.contract._c(), _Bates(self.params._c(), 0.0, 0.0, 0.0), CF_HESTON, 0)
+
+
+def cf_price_options(options: Sequence[CfOption], with_panels=False, with_kernel_ms=False):
+    """Prices of n options in one launch.  Returns the prices, plus the
+    (n, 2) panel counts of the P1 / P2 integrals and the GPU time in ms when
+    asked."""
+    n = len(options)
+    arr = (_CfOption * max(1, n))()
+    for i, o in enumerate(options):
+        arr[i] = o._c()
+    out = np.zeros(n)
+    panels = np.zeros((n, 2), dtype=np.uint32) if with_panels else None
+    kms = C.c_double()
+    _check(_cf_lib().ope_cf_price_options(arr, n, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                          panels.ctypes.data_as(C.POINTER(C.c_uint32)) if with_panels else None,
+                                          C.byref(kms)))
+    res: Tuple = (out,)
+    if with_panels:
+        res += (panels,)
+    if with_kernel_ms:
+        res += (kms.value,)
+    return res[0] if len(res) == 1 else res

@@ -1,4 +1,4 @@
-"""The C-ABI library loads, exports every symbol include/ope_mc.h declares,
+"""The C-ABI library loads, exports every symbol include/*.h declares,
 has the struct layout the header fixes, and validates arguments on the host
 (no GPU needed: invalid input is rejected before the device is touched)."""
 import ctypes as C
@@ -11,17 +11,46 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "ope_mc.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(ope_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in sorted(os.listdir(inc)):
+        if f.endswith(".h"):
+            text = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, f)).read(), flags=re.S)
+            names |= set(re.findall(r"\b(ope_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol(ope):
     L = ope.lib()
     names = declared_functions()
-    assert "ope_mc_heston" in names and "ope_mc_bates" in names
+    for n in ("ope_mc_heston", "ope_mc_bates", "ope_server_submit", "ope_heston_price", "ope_batch_bates"):
+        assert n in names
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_cf_struct_layout(ope):
+    from opemc import cf
+    assert C.sizeof(cf._Bates) == 40 + 24
+    assert C.sizeof(cf._Contract) == 6 * 8 + 8
+    assert C.sizeof(cf._CfOption) == 56 + 64 + 8
+
+
+@pytest.mark.parametrize("case", [
+    dict(spot=0.0), dict(strike=-1.0), dict(expiry=float("inf")), dict(rate=float("nan")),
+    dict(opt_type=5), dict(hp=dict(rho=float("nan"))),
+])
+def test_cf_invalid_arguments_rejected_on_host(ope, case):
+    kw = dict(spot=100.0, strike=100.0, expiry=1.0, rate=0.03, div_yield=0.0)
+    kw.update({k: v for k, v in case.items() if k in kw})
+    hp = ope._Heston(0.04, 1.5, 0.04, 0.5, -0.7)
+    for k, v in case.get("hp", {}).items():
+        setattr(hp, k, v)
+    out = C.c_double()
+    from opemc import cf
+    rc = cf._cf_lib().ope_heston_price(kw["spot"], kw["strike"], kw["expiry"], kw["rate"], kw["div_yield"],
+                                       C.byref(hp), case.get("opt_type", 0), C.byref(out))
+    assert rc == ope.OPE_ERR_INVALID_ARGUMENT
 
 
 def test_struct_layout(ope):
