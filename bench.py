@@ -19,9 +19,11 @@ One step is one pass of the hot path over one batch of synthetic input:
   counts and mixed payoffs in one launch.
 * serving: the persistent-kernel server under Poisson arrivals at 2000 req/s,
   each request run to a standard error of 1e-3 x price; p50/p99 latency.
+* analytic_cf: the GPU characteristic-function pricer on 32 parameter sets x
+  the 41 x 8 surface, Heston and Bates (20992 options per call).
 
-The default run also measures bates_surface, uneven_batch and serving and
-carries them in the same JSON line.
+The default run also measures bates_surface, uneven_batch, analytic_cf and
+serving and carries them in the same JSON line.
 
 The value is FP32-path mode path-steps/s (both antithetic siblings count);
 FP64 mode runs the same workload on identical Philox streams and the line
@@ -195,6 +197,71 @@ def secondary(ope, wm, name):
                               "frac": a64 / wm.PEAK_TFLOPS["f64"]}}
 
 
+def cf_options(ope, seed=11, n_sets=32):
+    """The analytic-pricer workload as an array of ope_cf_option records:
+    n_sets parameter sets, each priced on the 41-strike x 8-expiry surface
+    as Heston and as Bates: what a calibration sweep evaluates."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    rows = []
+    for _ in range(n_sets):
+        v0, theta = rng.uniform(0.02, 0.09, 2)
+        kappa, sigma, rho = rng.uniform(0.5, 4.0), rng.uniform(0.2, 1.0), rng.uniform(-0.9, -0.1)
+        for model in (0, 1):
+            lam, mj, sj = JUMPS if model else (0.0, 0.0, 0.0)
+            for t in EXPIRIES:
+                for k in STRIKES:
+                    rows.append((100.0, k, t, 0.03, 0.0, 0.0, 0, 0, v0, kappa, theta, sigma, rho, lam, mj, sj,
+                                 model, 0))
+    return np.array(rows, dtype=ope.CF_DTYPE)
+
+
+def analytic_cf(ope, wm, steps=5, with_cpu=True):
+    """GPU analytic prices per second, roofline against FP64, the oracle's
+    restatement of the reference pricer on the host cores beside it."""
+    import numpy as np
+
+    opts = cf_options(ope)
+    n = len(opts)
+    ope.cf_price_options(opts)  # warm-up
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        prices, panels, k = ope.cf_price_options(opts, with_panels=True, with_kernel_ms=True)
+        kms.append(k)
+    wall = (time.perf_counter() - t0) / steps
+    kernel_ms = sum(kms) / len(kms)
+    flops = sum(wm.cf_flops("bates" if m else "heston", int(p[0]), int(p[1])) for m, p in zip(opts["model"], panels))
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    desc = ("32 Heston parameter sets (numpy seed 11: v0, theta ~ U[0.02, 0.09], kappa ~ U[0.5, 4], sigma ~ "
+            "U[0.2, 1], rho ~ U[-0.9, -0.1]) x 41 call strikes 60..140 x 8 expiries 0.1..2, each as Heston and as "
+            f"Bates (lambda=0.5, mu_j=-0.1, sigma_j=0.15); S0=100, r=0.03, q=0; {n} options per call")
+    out = {"workload": desc, "options": n, "options_per_s": n / wall, "ms_per_call": wall * 1e3,
+           "kernel_ms": kernel_ms, "kernel_options_per_s": n / (kernel_ms * 1e-3),
+           "mean_panels_per_integral": float(panels.mean()),
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": wm.PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                        "frac": achieved / wm.PEAK_TFLOPS["f64"]},
+           "dtype": "f64"}
+    print(f"# analytic_cf: {n} options {wall * 1e3:.2f} ms/call, kernel {kernel_ms:.2f} ms",
+          file=sys.stderr, flush=True)
+    if with_cpu:
+        import oracle
+
+        cores = max(1, min(16, os.cpu_count() or 1))
+        args = np.stack([opts[f] for f in ("spot", "strike", "expiry", "rate", "div_yield", "v0", "kappa", "theta",
+                                           "sigma", "rho", "lambda_", "mu_j", "sigma_j")]
+                        + [opts["model"].astype(float), opts["opt_type"].astype(float)], axis=1)
+        t0 = time.perf_counter()
+        ref = oracle.cf_price_many(args, n_threads=cores)
+        cpu_wall = time.perf_counter() - t0
+        out["max_abs_diff_vs_oracle"] = float(np.max(np.abs(ref - prices)))
+        out["cpu_baseline"] = {"value": n / cpu_wall, "unit": "options/s", "cores": cores, "kind": "port",
+                               "sample": f"the same {n} options, C restatement of heston.rs / bates.rs, "
+                                         f"{cores} threads, {cpu_wall:.2f} s wall"}
+    return out
+
+
 def serving(ope, seconds):
     """The serving configuration (Poisson arrivals at 2000 req/s) in both
     precisions, plus the same requests all at once (saturated throughput)."""
@@ -263,7 +330,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="qe_asian_uao",
-                    choices=["qe_asian_uao", "bates_surface", "heston_euler", "uneven_batch", "serving"])
+                    choices=["qe_asian_uao", "bates_surface", "heston_euler", "uneven_batch", "serving",
+                             "analytic_cf"])
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the serving measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -302,6 +370,9 @@ def main():
           file=sys.stderr, flush=True)
     if args.workload == "serving":
         print(json.dumps({"serving": serving(ope, args.serve_seconds)}))
+        return
+    if args.workload == "analytic_cf":
+        print(json.dumps({"analytic_cf": analytic_cf(ope, wm, steps=args.steps, with_cpu=not args.no_cpu)}))
         return
     desc, conf, run, path_steps, flops = make_workload(ope, wm, args.workload)
 
@@ -350,6 +421,7 @@ def main():
     if args.workload == "qe_asian_uao" and not args.no_extra:
         for extra in ("bates_surface", "uneven_batch"):
             line[extra] = secondary(ope, wm, extra)
+        line["analytic_cf"] = analytic_cf(ope, wm, with_cpu=not args.no_cpu)
     if not args.no_serve:
         line["serving"] = serving(ope, args.serve_seconds)
     if not args.no_cpu:

@@ -41,7 +41,9 @@ __device__ __forceinline__ cpx cradd(cpx a, double s) { return C(a.re + s, a.im)
 __device__ __forceinline__ cpx crsub(double s, cpx a) { return C(s - a.re, -a.im); }      // f64 - Complex
 __device__ __forceinline__ cpx cexp_(cpx z) {
     const double e = exp(z.re);
-    return C(e * cos(z.im), e * sin(z.im));
+    double s, c;
+    sincos(z.im, &s, &c);  // one argument reduction for both
+    return C(e * c, e * s);
 }
 __device__ __forceinline__ cpx clog_(cpx z) { return C(log(hypot(z.re, z.im)), atan2(z.im, z.re)); }
 
@@ -173,65 +175,61 @@ __global__ __launch_bounds__(64) void k_cf(const CfDesc* __restrict__ opts, doub
     const cpx norm = is_p1 ? model_cf(C(0.0, -1.0), p) : C(1.0, 0.0);
     Panels& Q = P[half];
 
-    // the first panel, [0, 1]
-    if (hl < 15) fv[half][0][hl] = mapped(0.5 + 0.5 * kGkNodes[hl], p, x, is_p1, norm);
-    __syncthreads();
+    // Round 0 evaluates the first panel, [0, 1], as a lone left child; every
+    // later round splits the panel with the largest error into two children.
+    // One evaluation site keeps a single inlined copy of the CF code.
     if (hl == 0) {
-        double k0, e0;
-        panel_sums(fv[half][0], 0.0, 1.0, k0, e0);
-        Q.e[0] = e0;
-        Q.k[0] = k0;
-        Q.a[0] = 0.0;
-        Q.b[0] = 1.0;
-        s_n[half] = 1;
-        s_total[half] = k0;
-        s_err[half] = e0;
+        s_aw[half] = 0.0;
+        s_m[half] = 1.0;
+        s_bw[half] = 1.0;
     }
-    __syncthreads();
-
-    for (;;) {
-        const bool act0 = s_err[0] > kCfTol && s_n[0] < kCfMaxPanels;
-        const bool act1 = s_err[1] > kCfTol && s_n[1] < kCfMaxPanels;
-        if (!act0 && !act1) break;
-        const bool act = half == 0 ? act0 : act1;
-        if (act) {
-            // the panel with the largest error, the last one on ties
-            const int np = s_n[half];
-            double bv = -1.0;
-            int bi = -1;
-            for (int j = hl; j < np; j += 32) {
-                const double v = Q.e[j];
-                if (v >= bv) {
-                    bv = v;
-                    bi = j;
+    for (int round = 0;; ++round) {
+        bool act = true;
+        if (round > 0) {
+            const bool act0 = s_err[0] > kCfTol && s_n[0] < kCfMaxPanels;
+            const bool act1 = s_err[1] > kCfTol && s_n[1] < kCfMaxPanels;
+            if (!act0 && !act1) break;
+            act = half == 0 ? act0 : act1;
+            if (act) {
+                // the panel with the largest error, the last one on ties
+                const int np = s_n[half];
+                double bv = -1.0;
+                int bi = -1;
+                for (int j = hl; j < np; j += 32) {
+                    const double v = Q.e[j];
+                    if (v >= bv) {
+                        bv = v;
+                        bi = j;
+                    }
                 }
-            }
 #pragma unroll
-            for (int off = 16; off >= 1; off >>= 1) {
-                const double ov = __shfl_xor(bv, off, 32);
-                const int oi = __shfl_xor(bi, off, 32);
-                if (ov > bv || (ov == bv && oi > bi)) {
-                    bv = ov;
-                    bi = oi;
+                for (int off = 16; off >= 1; off >>= 1) {
+                    const double ov = __shfl_xor(bv, off, 32);
+                    const int oi = __shfl_xor(bi, off, 32);
+                    if (ov > bv || (ov == bv && oi > bi)) {
+                        bv = ov;
+                        bi = oi;
+                    }
                 }
-            }
-            if (hl == 0) {
-                const int w = bi, last = np - 1;
-                s_ew[half] = Q.e[w];
-                s_kw[half] = Q.k[w];
-                const double aw = Q.a[w], bw = Q.b[w];
-                Q.e[w] = Q.e[last];  // swap_remove
-                Q.k[w] = Q.k[last];
-                Q.a[w] = Q.a[last];
-                Q.b[w] = Q.b[last];
-                s_n[half] = last;
-                s_aw[half] = aw;
-                s_bw[half] = bw;
-                s_m[half] = 0.5 * (aw + bw);
+                if (hl == 0) {
+                    const int w = bi, last = np - 1;
+                    s_ew[half] = Q.e[w];
+                    s_kw[half] = Q.k[w];
+                    const double aw = Q.a[w], bw = Q.b[w];
+                    Q.e[w] = Q.e[last];  // swap_remove
+                    Q.k[w] = Q.k[last];
+                    Q.a[w] = Q.a[last];
+                    Q.b[w] = Q.b[last];
+                    s_n[half] = last;
+                    s_aw[half] = aw;
+                    s_bw[half] = bw;
+                    s_m[half] = 0.5 * (aw + bw);
+                }
             }
         }
         __syncthreads();
-        if (act && hl < 30) {
+        const int children = round == 0 ? 1 : 2;
+        if (act && hl < 15 * children) {
             const int c = hl >= 15 ? 1 : 0, i = hl - 15 * c;
             const double a = c ? s_m[half] : s_aw[half];
             const double b = c ? s_bw[half] : s_m[half];
@@ -239,27 +237,38 @@ __global__ __launch_bounds__(64) void k_cf(const CfDesc* __restrict__ opts, doub
             fv[half][c][i] = mapped(cc + h * kGkNodes[i], p, x, is_p1, norm);
         }
         __syncthreads();
-        if (act && hl < 2) {
+        if (act && hl < children) {
             const double a = hl ? s_m[half] : s_aw[half];
             const double b = hl ? s_bw[half] : s_m[half];
             panel_sums(fv[half][hl], a, b, s_kc[half][hl], s_ec[half][hl]);
         }
         __syncthreads();
         if (act && hl == 0) {
-            const double kl = s_kc[half][0], kr = s_kc[half][1], el = s_ec[half][0], er = s_ec[half][1];
-            s_total[half] += kl + kr - s_kw[half];
-            s_err[half] += el + er - s_ew[half];
-            int n = s_n[half];
-            Q.e[n] = el;
-            Q.k[n] = kl;
-            Q.a[n] = s_aw[half];
-            Q.b[n] = s_m[half];
-            ++n;
-            Q.e[n] = er;
-            Q.k[n] = kr;
-            Q.a[n] = s_m[half];
-            Q.b[n] = s_bw[half];
-            s_n[half] = n + 1;
+            const double kl = s_kc[half][0], el = s_ec[half][0];
+            if (round == 0) {
+                Q.e[0] = el;
+                Q.k[0] = kl;
+                Q.a[0] = 0.0;
+                Q.b[0] = 1.0;
+                s_n[half] = 1;
+                s_total[half] = kl;
+                s_err[half] = el;
+            } else {
+                const double kr = s_kc[half][1], er = s_ec[half][1];
+                s_total[half] += kl + kr - s_kw[half];
+                s_err[half] += el + er - s_ew[half];
+                int n = s_n[half];
+                Q.e[n] = el;
+                Q.k[n] = kl;
+                Q.a[n] = s_aw[half];
+                Q.b[n] = s_m[half];
+                ++n;
+                Q.e[n] = er;
+                Q.k[n] = kr;
+                Q.a[n] = s_m[half];
+                Q.b[n] = s_bw[half];
+                s_n[half] = n + 1;
+            }
         }
         __syncthreads();
     }

@@ -539,5 +539,5 @@ class Server:
 
 
 # ---------------------------------------------------------- analytic pricers
-from .cf import (BatesParams, CfOption, OptionContract, batch_bates, batch_heston,  # noqa: E402
+from .cf import (CF_DTYPE, BatesParams, CfOption, OptionContract, batch_bates, batch_heston,  # noqa: E402
                  bates_price, cf_price_options, heston_price)

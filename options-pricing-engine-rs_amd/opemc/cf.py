@@ -132,14 +132,30 @@ class CfOption:
         return _CfOption(self.contract._c(), _Bates(self.params._c(), 0.0, 0.0, 0.0), CF_HESTON, 0)
 
 
-def cf_price_options(options: Sequence[CfOption], with_panels=False, with_kernel_ms=False):
-    """Prices of n options in one launch.  Returns the prices, plus the
-    (n, 2) panel counts of the P1 / P2 integrals and the GPU time in ms when
-    asked."""
-    n = len(options)
-    arr = (_CfOption * max(1, n))()
-    for i, o in enumerate(options):
-        arr[i] = o._c()
+# numpy layout of ope_cf_option, for building large batches without a
+# Python object per option
+CF_DTYPE = np.dtype([("spot", "f8"), ("strike", "f8"), ("expiry", "f8"), ("rate", "f8"), ("div_yield", "f8"),
+                     ("vol", "f8"), ("opt_type", "i4"), ("reserved0", "i4"), ("v0", "f8"), ("kappa", "f8"),
+                     ("theta", "f8"), ("sigma", "f8"), ("rho", "f8"), ("lambda_", "f8"), ("mu_j", "f8"),
+                     ("sigma_j", "f8"), ("model", "i4"), ("reserved1", "i4")])
+assert CF_DTYPE.itemsize == C.sizeof(_CfOption)
+
+
+def cf_price_options(options, with_panels=False, with_kernel_ms=False):
+    """Prices of n options in one launch.  options: a sequence of CfOption,
+    or a numpy array of CF_DTYPE.  Returns the prices, plus the (n, 2) panel
+    counts of the P1 / P2 integrals and the GPU time in ms when asked."""
+    if isinstance(options, np.ndarray):
+        if options.dtype != CF_DTYPE:
+            raise TypeError("options array must have dtype CF_DTYPE")
+        n = len(options)
+        packed = np.ascontiguousarray(options)
+        arr = packed.ctypes.data_as(C.POINTER(_CfOption))
+    else:
+        n = len(options)
+        arr = (_CfOption * max(1, n))()
+        for i, o in enumerate(options):
+            arr[i] = o._c()
     out = np.zeros(n)
     panels = np.zeros((n, 2), dtype=np.uint32) if with_panels else None
     kms = C.c_double()

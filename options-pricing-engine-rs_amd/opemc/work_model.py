@@ -1,4 +1,4 @@
-"""Algorithmic work per path-step of the reference's Monte Carlo step body.
+"""Algorithmic work of the reference's Monte Carlo step body and analytic pricer.
 
 Counting rule: the floating-point operations of mc.rs's step body after loop
 invariants are hoisted; add / sub / mul / div / min / max / compare = 1,
@@ -17,6 +17,17 @@ giving W = 31 (Euler, Heston), 32.5 (Euler, Bates), 46 (QE, Heston),
 (intrinsic, discount, sum, sum of squares), plus 1 for the Asian mean or the
 barrier select.
 
+Analytic pricer (heston.rs:82-229, bates.rs:49-83), per integrand
+evaluation, same counting rule, complex product = 6, complex quotient = 11
+(num-complex's naive forms), complex exp = 5 (exp, cos, sin, 2 mul), complex
+ln = 3 (hypot, ln, atan2):
+  map u = (1-t)/t and its guards 7; stable_cf 150 (xi 11, d with the stable
+  square root 27, g 15, e^{-dt} 9, A 19, C 16, D 34, r phi i t 10, final
+  exp 9); P1 normalisation 11 on half the evaluations; e^{iux} cf 12;
+  division by iu 11
+giving W_CF = 185.5 (Heston), + 57 for the jump factor and its product =
+242.5 (Bates).  An integral with p panels evaluated 15 (2p - 1) nodes.
+
 MI355X vector peaks (datasheet): FP32 157.3 TFLOP/s (packed FP32 counted),
 FP64 78.6 TFLOP/s; HBM3E 8 TB/s.
 """
@@ -34,3 +45,11 @@ def flops_per_path(n_steps: int, scheme: str, jumps: bool, n_strikes: int = 1, p
 def flops(n_paths: int, n_steps: int, scheme: str, jumps: bool, n_strikes: int = 1, payoff: str = "european") -> float:
     """Algorithmic flops of one simulation of n_paths paths (both antithetic siblings counted)."""
     return n_paths * flops_per_path(n_steps, scheme, jumps, n_strikes, payoff)
+
+
+W_CF = {"heston": 185.5, "bates": 242.5}
+
+
+def cf_flops(model: str, panels_p1: int, panels_p2: int) -> float:
+    """Algorithmic flops of one analytic price from its quadrature panel counts."""
+    return W_CF[model] * 15.0 * ((2 * panels_p1 - 1) + (2 * panels_p2 - 1))

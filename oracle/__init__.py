@@ -95,6 +95,8 @@ def lib():
     L.ora_heston_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int]
     L.ora_bates_price.restype = C.c_double
     L.ora_bates_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston)] + [C.c_double] * 3 + [C.c_int]
+    L.ora_cf_price_many.restype = None
+    L.ora_cf_price_many.argtypes = [_dp, C.c_int64, _dp]
     L.ora_bsm_price.restype = C.c_double
     L.ora_bsm_price.argtypes = [C.c_double] * 6 + [C.c_int]
     L.ora_merton_price.restype = C.c_double
@@ -170,6 +172,28 @@ def heston_price(s, k, t, r, q, hp, opt_type=CALL):
 
 def bates_price(s, k, t, r, q, hp, lam, mu_j, sigma_j, opt_type=CALL):
     return lib().ora_bates_price(s, k, t, r, q, C.byref(hp), lam, mu_j, sigma_j, opt_type)
+
+
+def cf_price_many(args, n_threads=1):
+    """Analytic prices of the rows of args (n x 15: spot, strike, expiry,
+    rate, div_yield, v0, kappa, theta, sigma, rho, lambda, mu_j, sigma_j,
+    model 0 Heston / 1 Bates, opt_type), split over n_threads host threads
+    (the C loop runs without the GIL)."""
+    import concurrent.futures
+
+    a = np.ascontiguousarray(args, dtype=np.float64).reshape(-1, 15)
+    n = a.shape[0]
+    out = np.zeros(n)
+    bounds = np.linspace(0, n, max(1, n_threads) + 1).astype(int)
+
+    def run(k):
+        lo, hi = int(bounds[k]), int(bounds[k + 1])
+        if hi > lo:
+            lib().ora_cf_price_many(a[lo:hi], hi - lo, out[lo:hi])
+
+    with concurrent.futures.ThreadPoolExecutor(max(1, n_threads)) as ex:
+        list(ex.map(run, range(len(bounds) - 1)))
+    return out
 
 
 def bsm_price(s, k, t, r, q, vol, opt_type=CALL):

@@ -223,6 +223,19 @@ double ora_bates_price(double s, double k, double t, double r, double q, const o
     return call - s * exp(-q * t) + k * exp(-r * t);
 }
 
+/* n options, 15 doubles each: spot, strike, expiry, rate, div_yield, v0,
+ * kappa, theta, sigma, rho, lambda, mu_j, sigma_j, model (0 Heston, 1 Bates),
+ * opt_type; the loop a batch.rs chain runs per thread (batch.rs:23-42). */
+void ora_cf_price_many(const double *args, int64_t n, double *out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double *a = args + 15 * i;
+        ora_heston h = {a[5], a[6], a[7], a[8], a[9]};
+        const int opt = (int)a[14];
+        out[i] = a[13] != 0.0 ? ora_bates_price(a[0], a[1], a[2], a[3], a[4], &h, a[10], a[11], a[12], opt)
+                              : ora_heston_price(a[0], a[1], a[2], a[3], a[4], &h, opt);
+    }
+}
+
 double ora_ncdf(double x);
 
 /* bsm.rs:7-12, 55-60 */

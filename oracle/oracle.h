@@ -22,6 +22,7 @@
  *   ora_mc_price              mc.rs:148-207 (mc_heston 98-103 / mc_bates 107-113)
  *   ora_heston_price          heston.rs:61-229
  *   ora_bates_price           bates.rs:29-83
+ *   ora_cf_price_many         a chain of the two above (batch.rs:23-42)
  *   ora_bsm_price             bsm.rs:7-12, 55-60
  *   ora_merton_price          textbook Merton (1976) series (no reference counterpart)
  */

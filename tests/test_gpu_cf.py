@@ -135,3 +135,18 @@ def test_mc_matches_analytic_on_the_gpu(ope):
     cf = ope.bates_price(100.0, 100.0, 1.0, 0.03, 0.0, ope.BatesParams(p, 0.5, -0.1, 0.15))
     # Euler discretisation bias at 200 steps is well below 3 SE here
     assert abs(mc.price - cf) < 3.0 * mc.std_error + 0.02
+
+
+def test_record_array_input_is_the_object_input(ope):
+    hp = ope.HestonParams(0.05, 2.5, 0.03, 0.6, -0.4)
+    objs = [ope.CfOption(ope.OptionContract(100.0, k, t, 0.03, 0.01, 0.0, ope.OptionType(ot)),
+                         ope.BatesParams(hp, 0.8, -0.05, 0.2) if m else hp)
+            for m in (0, 1) for t in (0.3, 1.7) for k in (90.0, 110.0) for ot in (0, 1)]
+    recs = np.zeros(len(objs), dtype=ope.CF_DTYPE)
+    for i, o in enumerate(objs):
+        c, p = o.contract, o.params
+        h = p.heston if isinstance(p, ope.BatesParams) else p
+        recs[i] = (c.spot, c.strike, c.expiry, c.rate, c.div_yield, 0.0, int(c.opt_type), 0, h.v0, h.kappa, h.theta,
+                   h.sigma, h.rho, *((p.lambda_, p.mu_j, p.sigma_j) if isinstance(p, ope.BatesParams) else (0, 0, 0)),
+                   int(isinstance(p, ope.BatesParams)), 0)
+    assert np.array_equal(ope.cf_price_options(recs), ope.cf_price_options(objs))

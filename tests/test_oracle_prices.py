@@ -186,3 +186,15 @@ def test_baseline_cfg1_against_analytic(ora):
     assert count == 100_000
     assert 0.02 < se < 0.05
     assert abs(price - ora.heston_price(100, 100, 1, 0.03, 0, hp)) < 5 * se
+
+
+def test_cf_price_many_is_the_single_calls(ora):
+    rows, want = [], []
+    hp = ora.heston(0.04, 1.5, 0.04, 0.5, -0.7)
+    for model in (0, 1):
+        for k in (80.0, 100.0, 120.0):
+            rows.append([100.0, k, 0.75, 0.03, 0.01, 0.04, 1.5, 0.04, 0.5, -0.7, 0.5, -0.1, 0.15, model, k > 100])
+            want.append(ora.bates_price(100.0, k, 0.75, 0.03, 0.01, hp, 0.5, -0.1, 0.15, int(k > 100)) if model
+                        else ora.heston_price(100.0, k, 0.75, 0.03, 0.01, hp, int(k > 100)))
+    got = ora.cf_price_many(np.array(rows, dtype=float), n_threads=2)
+    assert list(got) == want
