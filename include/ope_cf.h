@@ -9,8 +9,17 @@
  *   ope_bates_price    <- bates::bates_price     (src/bates.rs:29-39)
  *   ope_batch_heston   <- batch::batch_heston    (src/batch.rs:23-27)
  *   ope_batch_bates    <- batch::batch_bates     (src/batch.rs:38-42)
+ *   ope_heston_price_and_greeks <- heston::heston_price_and_greeks (src/heston.rs:74-80)
+ *   ope_bates_price_and_greeks  <- bates::bates_price_and_greeks   (src/bates.rs:41-47)
+ *   ope_batch_heston_greeks     <- batch::batch_heston_greeks      (src/batch.rs:32-36)
+ *   ope_batch_bates_greeks      <- batch::batch_bates_greeks       (src/batch.rs:44-48)
  * plus ope_cf_price_options, which takes a model per option (what a
  * calibration's residual and Jacobian evaluations need).
+ *
+ * Greeks are the reference's bump-and-reprice (greeks.rs:19-82): 14 prices
+ * per option (spot +-1 %, sqrt(v0) +-1 and +-2 vol points, expiry - 1 day,
+ * rate +-1 bp and the cross bumps), all of a batch's bumped prices in one
+ * launch, combined on the host with the reference's difference formulas.
  *
  * The quadrature is the reference's: Gil-Pelaez inversion of the Albrecher
  * form CF (heston.rs:82-127, bates.rs:49-83), globally adaptive
@@ -62,6 +71,18 @@ typedef struct ope_cf_option {
 
 enum { OPE_CF_HESTON = 0, OPE_CF_BATES = 1 };
 
+/* types.rs:29-39 */
+typedef struct ope_pricing_result {
+    double price;
+    double delta;
+    double gamma;
+    double vega;  /* per unit of volatility sqrt(v0) */
+    double theta; /* (price(T - 1 day) - price) / 1 day */
+    double rho;
+    double vanna;
+    double volga;
+} ope_pricing_result;
+
 /* heston.rs:61-72 */
 int ope_heston_price(double spot, double strike, double expiry, double rate, double div_yield,
                      const ope_heston_params *params, int32_t opt_type, double *out);
@@ -72,6 +93,18 @@ int ope_bates_price(double spot, double strike, double expiry, double rate, doub
 int ope_batch_heston(const ope_option_contract *contracts, size_t n, const ope_heston_params *params, double *out);
 /* batch.rs:38-42 */
 int ope_batch_bates(const ope_option_contract *contracts, size_t n, const ope_bates_params *params, double *out);
+/* heston.rs:74-80 */
+int ope_heston_price_and_greeks(double spot, double strike, double expiry, double rate, double div_yield,
+                                const ope_heston_params *params, int32_t opt_type, ope_pricing_result *out);
+/* bates.rs:41-47 */
+int ope_bates_price_and_greeks(double spot, double strike, double expiry, double rate, double div_yield,
+                               const ope_bates_params *params, int32_t opt_type, ope_pricing_result *out);
+/* batch.rs:32-36 */
+int ope_batch_heston_greeks(const ope_option_contract *contracts, size_t n, const ope_heston_params *params,
+                            ope_pricing_result *out);
+/* batch.rs:44-48 */
+int ope_batch_bates_greeks(const ope_option_contract *contracts, size_t n, const ope_bates_params *params,
+                           ope_pricing_result *out);
 /* n options with their own models in one launch.  panels, when given,
  * receives 2 entries per option (panels used by the P1 and P2 integrals);
  * kernel_ms, when given, the GPU time of the launch. */

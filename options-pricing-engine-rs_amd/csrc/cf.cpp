@@ -115,6 +115,64 @@ int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, doub
     return OPE_OK;
 }
 
+// greeks.rs:19-82: the 14 bumped prices of one option, in this order.
+constexpr int kBumps = 14;
+
+void bump_descs(const CfDesc& base, CfDesc* d) {
+    const double ds = 0.01 * base.spot, dv = 0.01, dr = 1e-4, dt = 1.0 / 365.0;
+    const double v_cur = std::sqrt(base.v0);
+    const double vu = v_cur + dv, vd = std::fmax(v_cur - dv, 1e-8);
+    const double vu2 = v_cur + 2.0 * dv, vd2 = std::fmax(v_cur - 2.0 * dv, 1e-8);
+    for (int i = 0; i < kBumps; ++i) d[i] = base;
+    d[1].spot = base.spot + ds;  // delta, gamma
+    d[2].spot = base.spot - ds;
+    d[3].v0 = vu * vu;  // vega
+    d[4].v0 = vd * vd;
+    d[5].expiry = std::fmax(base.expiry - dt, 1e-6);  // theta
+    d[6].rate = base.rate + dr;  // rho
+    d[7].rate = base.rate - dr;
+    d[8].v0 = vu * vu;  // vanna: delta at v+dv and at v-dv
+    d[8].spot = base.spot + ds;
+    d[9].v0 = vu * vu;
+    d[9].spot = base.spot - ds;
+    d[10].v0 = vd * vd;
+    d[10].spot = base.spot + ds;
+    d[11].v0 = vd * vd;
+    d[11].spot = base.spot - ds;
+    d[12].v0 = vu2 * vu2;  // volga
+    d[13].v0 = vd2 * vd2;
+}
+
+ope_pricing_result combine(const CfDesc& base, const double* p) {
+    const double ds = 0.01 * base.spot, dv = 0.01, dr = 1e-4, dt = 1.0 / 365.0;
+    ope_pricing_result g;
+    const double price = p[0];
+    g.price = price;
+    g.delta = (p[1] - p[2]) / (2.0 * ds);
+    g.gamma = (p[1] - 2.0 * price + p[2]) / (ds * ds);
+    g.vega = (p[3] - p[4]) / (2.0 * dv);
+    g.theta = (p[5] - price) / dt;
+    g.rho = (p[6] - p[7]) / (2.0 * dr);
+    const double delta_vup = (p[8] - p[9]) / (2.0 * ds);
+    const double delta_vdn = (p[10] - p[11]) / (2.0 * ds);
+    g.vanna = (delta_vup - delta_vdn) / (2.0 * dv);
+    const double vega_up = (p[12] - price) / (2.0 * dv);
+    const double vega_dn = (price - p[13]) / (2.0 * dv);
+    g.volga = (vega_up - vega_dn) / (2.0 * dv);
+    return g;
+}
+
+int run_greeks(const std::vector<CfDesc>& base, ope_pricing_result* out) {
+    const size_t n = base.size();
+    if (n > kMaxOptions / kBumps) return fail(OPE_ERR_INVALID_ARGUMENT, "too many options in one call");
+    std::vector<CfDesc> all(n * kBumps);
+    for (size_t i = 0; i < n; ++i) bump_descs(base[i], &all[i * kBumps]);
+    std::vector<double> px(all.size());
+    if (int rc = run_cf(all, px.data(), nullptr, nullptr)) return rc;
+    for (size_t i = 0; i < n; ++i) out[i] = combine(base[i], &px[i * kBumps]);
+    return OPE_OK;
+}
+
 ope_option_contract contract(double spot, double strike, double expiry, double rate, double div_yield,
                              int32_t opt_type) {
     ope_option_contract k{};
@@ -168,6 +226,47 @@ int ope_batch_bates(const ope_option_contract* contracts, size_t n, const ope_ba
     for (size_t i = 0; i < n; ++i)
         if (int rc = make_cf_desc(contracts[i], *params, true, d[i])) return fail(rc, "option %zu: %s", i, g_last_error.c_str());
     return run_cf(d, out, nullptr, nullptr);
+}
+
+int ope_heston_price_and_greeks(double spot, double strike, double expiry, double rate, double div_yield,
+                                const ope_heston_params* params, int32_t opt_type, ope_pricing_result* out) {
+    if (!params || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    ope_bates_params bp{};
+    bp.heston = *params;
+    std::vector<CfDesc> d(1);
+    if (int rc = make_cf_desc(contract(spot, strike, expiry, rate, div_yield, opt_type), bp, false, d[0])) return rc;
+    return run_greeks(d, out);
+}
+
+int ope_bates_price_and_greeks(double spot, double strike, double expiry, double rate, double div_yield,
+                               const ope_bates_params* params, int32_t opt_type, ope_pricing_result* out) {
+    if (!params || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<CfDesc> d(1);
+    if (int rc = make_cf_desc(contract(spot, strike, expiry, rate, div_yield, opt_type), *params, true, d[0]))
+        return rc;
+    return run_greeks(d, out);
+}
+
+int ope_batch_heston_greeks(const ope_option_contract* contracts, size_t n, const ope_heston_params* params,
+                            ope_pricing_result* out) {
+    if (n == 0) return OPE_OK;
+    if (!contracts || !params || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    ope_bates_params bp{};
+    bp.heston = *params;
+    std::vector<CfDesc> d(n);
+    for (size_t i = 0; i < n; ++i)
+        if (int rc = make_cf_desc(contracts[i], bp, false, d[i])) return fail(rc, "option %zu: %s", i, g_last_error.c_str());
+    return run_greeks(d, out);
+}
+
+int ope_batch_bates_greeks(const ope_option_contract* contracts, size_t n, const ope_bates_params* params,
+                           ope_pricing_result* out) {
+    if (n == 0) return OPE_OK;
+    if (!contracts || !params || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<CfDesc> d(n);
+    for (size_t i = 0; i < n; ++i)
+        if (int rc = make_cf_desc(contracts[i], *params, true, d[i])) return fail(rc, "option %zu: %s", i, g_last_error.c_str());
+    return run_greeks(d, out);
 }
 
 int ope_cf_price_options(const ope_cf_option* options, size_t n, double* out, uint32_t* panels, double* kernel_ms) {

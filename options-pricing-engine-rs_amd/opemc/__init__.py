@@ -539,5 +539,6 @@ class Server:
 
 
 # ---------------------------------------------------------- analytic pricers
-from .cf import (CF_DTYPE, BatesParams, CfOption, OptionContract, batch_bates, batch_heston,  # noqa: E402
-                 bates_price, cf_price_options, heston_price)
+from .cf import (CF_DTYPE, BatesParams, CfOption, OptionContract, PricingResult, batch_bates,  # noqa: E402
+                 batch_bates_greeks, batch_heston, batch_heston_greeks, bates_price, bates_price_and_greeks,
+                 cf_price_options, heston_price, heston_price_and_greeks)

@@ -34,6 +34,27 @@ class _CfOption(C.Structure):
     _fields_ = [("contract", _Contract), ("params", _Bates), ("model", C.c_int32), ("reserved_", C.c_int32)]
 
 
+class _PricingResult(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("price", "delta", "gamma", "vega", "theta", "rho", "vanna", "volga")]
+
+
+@dataclasses.dataclass(frozen=True)
+class PricingResult:
+    """types.rs:29-39"""
+    price: float
+    delta: float
+    gamma: float
+    vega: float
+    theta: float
+    rho: float
+    vanna: float
+    volga: float
+
+    @classmethod
+    def _from(cls, r):
+        return cls(r.price, r.delta, r.gamma, r.vega, r.theta, r.rho, r.vanna, r.volga)
+
+
 def _cf_lib():
     L = lib()
     if not getattr(L, "_cf_ready", False):
@@ -44,8 +65,14 @@ def _cf_lib():
         L.ope_batch_bates.argtypes = [C.POINTER(_Contract), C.c_size_t, C.POINTER(_Bates), C.POINTER(d)]
         L.ope_cf_price_options.argtypes = [C.POINTER(_CfOption), C.c_size_t, C.POINTER(d),
                                            C.POINTER(C.c_uint32), C.POINTER(d)]
+        R = C.POINTER(_PricingResult)
+        L.ope_heston_price_and_greeks.argtypes = [d, d, d, d, d, C.POINTER(_Heston), C.c_int32, R]
+        L.ope_bates_price_and_greeks.argtypes = [d, d, d, d, d, C.POINTER(_Bates), C.c_int32, R]
+        L.ope_batch_heston_greeks.argtypes = [C.POINTER(_Contract), C.c_size_t, C.POINTER(_Heston), R]
+        L.ope_batch_bates_greeks.argtypes = [C.POINTER(_Contract), C.c_size_t, C.POINTER(_Bates), R]
         for n in ("ope_heston_price", "ope_bates_price", "ope_batch_heston", "ope_batch_bates",
-                  "ope_cf_price_options"):
+                  "ope_cf_price_options", "ope_heston_price_and_greeks", "ope_bates_price_and_greeks",
+                  "ope_batch_heston_greeks", "ope_batch_bates_greeks"):
             getattr(L, n).restype = C.c_int
         L._cf_ready = True
     return L
@@ -118,6 +145,42 @@ def batch_bates(contracts: Sequence[OptionContract], params: BatesParams) -> np.
     _check(_cf_lib().ope_batch_bates(_contracts(contracts), len(contracts), C.byref(bp),
                                      out.ctypes.data_as(C.POINTER(C.c_double))))
     return out
+
+
+def heston_price_and_greeks(spot, strike, expiry, rate, div_yield, params: HestonParams,
+                            opt_type: OptionType = OptionType.Call) -> PricingResult:
+    """heston.rs:74-80, bump-and-reprice (greeks.rs:19-82) on the GPU."""
+    out = _PricingResult()
+    hp = params._c()
+    _check(_cf_lib().ope_heston_price_and_greeks(spot, strike, expiry, rate, div_yield, C.byref(hp), int(opt_type),
+                                                 C.byref(out)))
+    return PricingResult._from(out)
+
+
+def bates_price_and_greeks(spot, strike, expiry, rate, div_yield, params: BatesParams,
+                           opt_type: OptionType = OptionType.Call) -> PricingResult:
+    """bates.rs:41-47"""
+    out = _PricingResult()
+    bp = params._c()
+    _check(_cf_lib().ope_bates_price_and_greeks(spot, strike, expiry, rate, div_yield, C.byref(bp), int(opt_type),
+                                                C.byref(out)))
+    return PricingResult._from(out)
+
+
+def batch_heston_greeks(contracts: Sequence[OptionContract], params: HestonParams):
+    """batch.rs:32-36: one launch for every bumped price of the chain."""
+    out = (_PricingResult * max(1, len(contracts)))()
+    hp = params._c()
+    _check(_cf_lib().ope_batch_heston_greeks(_contracts(contracts), len(contracts), C.byref(hp), out))
+    return [PricingResult._from(out[i]) for i in range(len(contracts))]
+
+
+def batch_bates_greeks(contracts: Sequence[OptionContract], params: BatesParams):
+    """batch.rs:44-48"""
+    out = (_PricingResult * max(1, len(contracts)))()
+    bp = params._c()
+    _check(_cf_lib().ope_batch_bates_greeks(_contracts(contracts), len(contracts), C.byref(bp), out))
+    return [PricingResult._from(out[i]) for i in range(len(contracts))]
 
 
 @dataclasses.dataclass(frozen=True)

@@ -95,6 +95,8 @@ def lib():
     L.ora_heston_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int]
     L.ora_bates_price.restype = C.c_double
     L.ora_bates_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston)] + [C.c_double] * 3 + [C.c_int]
+    L.ora_cf_greeks.restype = None
+    L.ora_cf_greeks.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int] + [C.c_double] * 3 + [C.c_int, _dp]
     L.ora_cf_price_many.restype = None
     L.ora_cf_price_many.argtypes = [_dp, C.c_int64, _dp]
     L.ora_bsm_price.restype = C.c_double
@@ -172,6 +174,19 @@ def heston_price(s, k, t, r, q, hp, opt_type=CALL):
 
 def bates_price(s, k, t, r, q, hp, lam, mu_j, sigma_j, opt_type=CALL):
     return lib().ora_bates_price(s, k, t, r, q, C.byref(hp), lam, mu_j, sigma_j, opt_type)
+
+
+GREEK_NAMES = ("price", "delta", "gamma", "vega", "theta", "rho", "vanna", "volga")
+
+
+def cf_greeks(s, k, t, r, q, hp, jumps=None, opt_type=CALL):
+    """greeks.rs bump-and-reprice through heston_price (jumps None) or
+    bates_price (jumps = (lambda, mu_j, sigma_j)); a dict of the 8 fields of
+    PricingResult."""
+    out = np.zeros(8)
+    lam, mj, sj = jumps if jumps is not None else (0.0, 0.0, 0.0)
+    lib().ora_cf_greeks(s, k, t, r, q, C.byref(hp), int(jumps is not None), lam, mj, sj, opt_type, out)
+    return dict(zip(GREEK_NAMES, out))
 
 
 def cf_price_many(args, n_threads=1):

@@ -236,6 +236,48 @@ void ora_cf_price_many(const double *args, int64_t n, double *out) {
     }
 }
 
+/* greeks.rs:19-82: bump-and-reprice Greeks through the CF pricers, 14 prices
+ * per option (dS = 1 % of spot, dv = 1 vol point on sqrt(v0), dr = 1 bp,
+ * dt = 1 day).  out: price, delta, gamma, vega, theta, rho, vanna, volga. */
+static double cf_price(double s, double k, double t, double r, double q, const ora_heston *p, int bates,
+                       double lambda, double mu_j, double sigma_j, int opt) {
+    return bates ? ora_bates_price(s, k, t, r, q, p, lambda, mu_j, sigma_j, opt) : ora_heston_price(s, k, t, r, q, p, opt);
+}
+
+static ora_heston with_v0(const ora_heston *p, double v0) {
+    ora_heston h = *p;
+    h.v0 = v0;
+    return h;
+}
+
+void ora_cf_greeks(double s, double k, double t, double r, double q, const ora_heston *p, int bates, double lambda,
+                   double mu_j, double sigma_j, int opt, double *out) {
+#define PX(S, K, T, R, Q, P) cf_price(S, K, T, R, Q, P, bates, lambda, mu_j, sigma_j, opt)
+    double price = PX(s, k, t, r, q, p);
+    double ds = 0.01 * s, dv = 0.01, dr = 1e-4, dt = 1.0 / 365.0;
+    double pu = PX(s + ds, k, t, r, q, p), pd = PX(s - ds, k, t, r, q, p);
+    double delta = (pu - pd) / (2.0 * ds);
+    double gamma = (pu - 2.0 * price + pd) / (ds * ds);
+    double v_cur = sqrt(p->v0);
+    double vu = v_cur + dv, vd = fmax(v_cur - dv, 1e-8);
+    ora_heston p_vup = with_v0(p, vu * vu), p_vdn = with_v0(p, vd * vd);
+    double vega = (PX(s, k, t, r, q, &p_vup) - PX(s, k, t, r, q, &p_vdn)) / (2.0 * dv);
+    double t_dn = fmax(t - dt, 1e-6);
+    double theta = (PX(s, k, t_dn, r, q, p) - price) / dt;
+    double rho = (PX(s, k, t, r + dr, q, p) - PX(s, k, t, r - dr, q, p)) / (2.0 * dr);
+    double delta_vup = (PX(s + ds, k, t, r, q, &p_vup) - PX(s - ds, k, t, r, q, &p_vup)) / (2.0 * ds);
+    double delta_vdn = (PX(s + ds, k, t, r, q, &p_vdn) - PX(s - ds, k, t, r, q, &p_vdn)) / (2.0 * ds);
+    double vanna = (delta_vup - delta_vdn) / (2.0 * dv);
+    double vu2 = v_cur + 2.0 * dv, vd2 = fmax(v_cur - 2.0 * dv, 1e-8);
+    ora_heston p_vup2 = with_v0(p, vu2 * vu2), p_vdn2 = with_v0(p, vd2 * vd2);
+    double vega_up = (PX(s, k, t, r, q, &p_vup2) - price) / (2.0 * dv);
+    double vega_dn = (price - PX(s, k, t, r, q, &p_vdn2)) / (2.0 * dv);
+    double volga = (vega_up - vega_dn) / (2.0 * dv);
+#undef PX
+    out[0] = price; out[1] = delta; out[2] = gamma; out[3] = vega;
+    out[4] = theta; out[5] = rho; out[6] = vanna; out[7] = volga;
+}
+
 double ora_ncdf(double x);
 
 /* bsm.rs:7-12, 55-60 */

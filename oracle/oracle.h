@@ -23,6 +23,7 @@
  *   ora_heston_price          heston.rs:61-229
  *   ora_bates_price           bates.rs:29-83
  *   ora_cf_price_many         a chain of the two above (batch.rs:23-42)
+ *   ora_cf_greeks             greeks.rs:19-82 bump-and-reprice over the two above
  *   ora_bsm_price             bsm.rs:7-12, 55-60
  *   ora_merton_price          textbook Merton (1976) series (no reference counterpart)
  */

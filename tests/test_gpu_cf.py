@@ -150,3 +150,57 @@ def test_record_array_input_is_the_object_input(ope):
                    h.sigma, h.rho, *((p.lambda_, p.mu_j, p.sigma_j) if isinstance(p, ope.BatesParams) else (0, 0, 0)),
                    int(isinstance(p, ope.BatesParams)), 0)
     assert np.array_equal(ope.cf_price_options(recs), ope.cf_price_options(objs))
+
+
+GREEKS = ("price", "delta", "gamma", "vega", "theta", "rho", "vanna", "volga")
+
+
+@pytest.mark.parametrize("model", ["heston", "bates"])
+def test_greeks_match_the_oracle(ope, ora, model):
+    # greeks.rs:19-82 through the GPU pricer against the same bump-and-reprice
+    # through the oracle's pricer: the 14 bumped prices agree to ~5e-14, the
+    # difference quotients divide that by at most (2 dv)^2 = 4e-4
+    ps = PARAM_SETS[1]
+    hp, ohp = ope.HestonParams(**ps), ora.heston(**ps)
+    chain = [ope.OptionContract(100.0, k, t, 0.03, q, 0.0, ope.OptionType(ot))
+             for k in (80.0, 100.0, 125.0) for t in (0.1, 1.0, 3.0) for q in (0.0, 0.02) for ot in (0, 1)]
+    if model == "heston":
+        got = ope.batch_heston_greeks(chain, hp)
+    else:
+        got = ope.batch_bates_greeks(chain, ope.BatesParams(hp, 0.5, -0.1, 0.15))
+    worst = 0.0
+    for c, g in zip(chain, got):
+        want = ora.cf_greeks(c.spot, c.strike, c.expiry, c.rate, c.div_yield, ohp,
+                             None if model == "heston" else (0.5, -0.1, 0.15), int(c.opt_type))
+        for name in GREEKS:
+            err = abs(getattr(g, name) - want[name])
+            worst = max(worst, err)
+            assert err < 1e-8 * max(1.0, abs(want[name])), (c, name, getattr(g, name), want[name])
+    print(f"greeks {model}: max |gpu - oracle| = {worst:.2e}", flush=True)
+
+
+def test_greeks_single_calls_equal_the_batch(ope):
+    hp = ope.HestonParams(0.04, 2.0, 0.04, 0.3, -0.7)
+    bp = ope.BatesParams(hp, 0.5, -0.1, 0.15)
+    chain = [ope.OptionContract(100.0, k, 1.0, 0.05, 0.0) for k in (90.0, 100.0, 110.0)]
+    for c, g in zip(chain, ope.batch_heston_greeks(chain, hp)):
+        assert g == ope.heston_price_and_greeks(c.spot, c.strike, c.expiry, c.rate, c.div_yield, hp)
+    for c, g in zip(chain, ope.batch_bates_greeks(chain, bp)):
+        assert g == ope.bates_price_and_greeks(c.spot, c.strike, c.expiry, c.rate, c.div_yield, bp)
+        assert g.price == ope.bates_price(c.spot, c.strike, c.expiry, c.rate, c.div_yield, bp)
+
+
+def test_reference_greeks_signs(ope):
+    # heston.rs:322-339, bates.rs:118-136
+    p = ope.HestonParams(0.04, 2.0, 0.04, 0.3, -0.7)
+    bp = ope.BatesParams(p, 0.5, -0.1, 0.15)
+    for r in (ope.heston_price_and_greeks(100.0, 100.0, 1.0, 0.05, 0.0, p),
+              ope.bates_price_and_greeks(100.0, 100.0, 1.0, 0.05, 0.0, bp)):
+        assert 0.0 < r.delta < 1.0 and r.gamma > 0.0 and r.vega > 0.0 and r.rho > 0.0
+    for r in (ope.heston_price_and_greeks(100.0, 100.0, 1.0, 0.05, 0.0, p, ope.OptionType.Put),
+              ope.bates_price_and_greeks(100.0, 100.0, 1.0, 0.05, 0.0, bp, ope.OptionType.Put)):
+        assert -1.0 < r.delta < 0.0 and r.gamma > 0.0
+    # bates.rs:140-148: without jumps the Greeks follow Heston's
+    b = ope.bates_price_and_greeks(100.0, 100.0, 1.0, 0.05, 0.0, ope.BatesParams(p, 0.0, 0.0, 1e-8))
+    h = ope.heston_price_and_greeks(100.0, 100.0, 1.0, 0.05, 0.0, p)
+    assert abs(b.delta - h.delta) < 0.01 and abs(b.vega - h.vega) < 1.0

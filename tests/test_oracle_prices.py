@@ -198,3 +198,40 @@ def test_cf_price_many_is_the_single_calls(ora):
                         else ora.heston_price(100.0, k, 0.75, 0.03, 0.01, hp, int(k > 100)))
     got = ora.cf_price_many(np.array(rows, dtype=float), n_threads=2)
     assert list(got) == want
+
+
+def _bsm_greeks(s, k, t, r, q, v):
+    """Closed-form BSM price and Greeks (per unit vol and rate; theta = -dP/dT)."""
+    from scipy.stats import norm
+    sq = math.sqrt(t)
+    d1 = (math.log(s / k) + (r - q + 0.5 * v * v) * t) / (v * sq)
+    d2 = d1 - v * sq
+    eq, er = math.exp(-q * t), math.exp(-r * t)
+    vega = s * eq * norm.pdf(d1) * sq
+    return dict(price=s * eq * norm.cdf(d1) - k * er * norm.cdf(d2), delta=eq * norm.cdf(d1),
+                gamma=eq * norm.pdf(d1) / (s * v * sq), vega=vega,
+                theta=-s * eq * norm.pdf(d1) * v / (2 * sq) + q * s * eq * norm.cdf(d1) - r * k * er * norm.cdf(d2),
+                rho=k * t * er * norm.cdf(d2), vanna=-eq * norm.pdf(d1) * d2 / v, volga=vega * d1 * d2 / v)
+
+
+@pytest.mark.parametrize("k,t", [(105.0, 1.0), (90.0, 0.5)])
+def test_cf_greeks_reduce_to_bsm(ora, k, t):
+    # greeks.rs:19-82 on a Heston model with (almost) constant variance: the
+    # bump-and-reprice Greeks approach the closed-form BSM Greeks up to their
+    # finite-difference truncation (ds = 1 % of spot, dv = 1 vol point, dt = 1 day)
+    g = ora.cf_greeks(100.0, k, t, 0.03, 0.0, ora.heston(0.04, 1e-6, 0.04, 1e-5, 0.0))
+    b = _bsm_greeks(100.0, k, t, 0.03, 0.0, 0.2)
+    tol = dict(price=1e-6, delta=5e-4, gamma=5e-5, vega=5e-3, theta=2e-2, rho=1e-4, vanna=2e-3, volga=0.1)
+    for name, tl in tol.items():
+        assert abs(g[name] - b[name]) < tl * max(1.0, abs(b[name])), (name, g[name], b[name])
+
+
+def test_cf_greeks_reference_signs(ora):
+    # heston.rs greeks_signs / greeks_put_delta_negative, bates.rs greeks_match_heston_no_jumps
+    hp = ora.heston(0.04, 2.0, 0.04, 0.3, -0.7)
+    c = ora.cf_greeks(100.0, 100.0, 1.0, 0.05, 0.0, hp)
+    assert 0.0 < c["delta"] < 1.0 and c["gamma"] > 0 and c["vega"] > 0 and c["rho"] > 0
+    p = ora.cf_greeks(100.0, 100.0, 1.0, 0.05, 0.0, hp, opt_type=ora.PUT)
+    assert -1.0 < p["delta"] < 0.0 and p["gamma"] > 0
+    b = ora.cf_greeks(100.0, 100.0, 1.0, 0.05, 0.0, hp, jumps=(0.0, 0.0, 1e-8))
+    assert abs(b["delta"] - c["delta"]) < 0.01 and abs(b["vega"] - c["vega"]) < 1.0
