@@ -1,7 +1,8 @@
 /*
- * ope_cf.h -- analytic Heston / Bates European prices on the GPU by
- * characteristic-function inversion: the reference's heston_price /
- * bates_price and their chain batches, one wavefront per option.
+ * ope_cf.h -- the reference's analytic pricers on the GPU: Heston / Bates
+ * European prices by characteristic-function inversion (one wavefront per
+ * option), their bump-and-reprice Greeks, Black-Scholes-Merton prices and
+ * Greeks and implied volatilities (one lane per option), single and chain.
  *
  * Drop-in replacements (same arguments and meaning; a status code and an
  * output pointer instead of the bare f64):
@@ -13,8 +14,20 @@
  *   ope_bates_price_and_greeks  <- bates::bates_price_and_greeks   (src/bates.rs:41-47)
  *   ope_batch_heston_greeks     <- batch::batch_heston_greeks      (src/batch.rs:32-36)
  *   ope_batch_bates_greeks      <- batch::batch_bates_greeks       (src/batch.rs:44-48)
+ *   ope_bsm_price_and_greeks    <- bsm::bsm_price_and_greeks       (src/bsm.rs:11-35)
+ *   ope_bsm_price               <- bsm::bsm_price                  (src/bsm.rs:49-54)
+ *   ope_implied_vol             <- iv::implied_vol                 (src/iv.rs:14-20)
+ *   ope_batch_bsm               <- batch::batch_bsm                (src/batch.rs:13-15)
+ *   ope_batch_bsm_price         <- batch::batch_bsm_price          (src/batch.rs:18-20)
+ *   ope_batch_implied_vol       <- batch::batch_implied_vol        (src/batch.rs:51-56)
  * plus ope_cf_price_options, which takes a model per option (what a
- * calibration's residual and Jacobian evaluations need).
+ * calibration's residual and Jacobian evaluations need), and the _ex chain
+ * variants that also report the GPU time.
+ *
+ * BSM and implied volatility run one lane per option with the reference's
+ * expressions, including its theta, whose carry term has the opposite sign of
+ * -dPrice/dT (bsm.rs:38-45).  An implied volatility the reference returns as
+ * None comes back as found = 0 and vol = NaN.
  *
  * Greeks are the reference's bump-and-reprice (greeks.rs:19-82): 14 prices
  * per option (spot +-1 %, sqrt(v0) +-1 and +-2 vol points, expiry - 1 day,
@@ -30,8 +43,9 @@
  * drift carries r only while the spot is discounted at q (heston.rs:126).
  *
  * Errors: OPE_ERR_INVALID_ARGUMENT for a null pointer, a non-finite
- * argument, or spot / strike <= 0 (ln(S/K) undefined); OPE_ERR_NO_DEVICE
- * without a GPU.  There is no CPU fallback.
+ * contract or model argument, an unknown option type, or (CF pricers)
+ * spot / strike <= 0; OPE_ERR_NO_DEVICE without a GPU.  There is no CPU
+ * fallback.
  */
 #ifndef OPE_CF_H
 #define OPE_CF_H
@@ -105,6 +119,25 @@ int ope_batch_heston_greeks(const ope_option_contract *contracts, size_t n, cons
 /* batch.rs:44-48 */
 int ope_batch_bates_greeks(const ope_option_contract *contracts, size_t n, const ope_bates_params *params,
                            ope_pricing_result *out);
+/* bsm.rs:11-35 (vol = contract.vol) */
+int ope_bsm_price_and_greeks(const ope_option_contract *contract, ope_pricing_result *out);
+/* bsm.rs:49-54 */
+int ope_bsm_price(const ope_option_contract *contract, double *out);
+/* iv.rs:14-20 (contract.vol is ignored) */
+int ope_implied_vol(const ope_option_contract *contract, double market_price, double *vol, uint8_t *found);
+/* batch.rs:13-15 */
+int ope_batch_bsm(const ope_option_contract *contracts, size_t n, ope_pricing_result *out);
+/* batch.rs:18-20 */
+int ope_batch_bsm_price(const ope_option_contract *contracts, size_t n, double *out);
+/* batch.rs:51-56 */
+int ope_batch_implied_vol(const ope_option_contract *contracts, const double *market_prices, size_t n, double *vol,
+                          uint8_t *found);
+/* One launch over a chain: prices (out_price) or prices and Greeks (out_greeks),
+ * whichever is non-null (out_greeks wins); kernel_ms, when given, the GPU time. */
+int ope_batch_bsm_ex(const ope_option_contract *contracts, size_t n, double *out_price, ope_pricing_result *out_greeks,
+                     double *kernel_ms);
+int ope_batch_implied_vol_ex(const ope_option_contract *contracts, const double *market_prices, size_t n,
+                             double *vol, uint8_t *found, double *kernel_ms);
 /* n options with their own models in one launch.  panels, when given,
  * receives 2 entries per option (panels used by the P1 and P2 integrals);
  * kernel_ms, when given, the GPU time of the launch. */

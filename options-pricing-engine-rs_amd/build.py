@@ -34,6 +34,7 @@ UNITS = [
     ("mc_server.hip", ["-ffp-contract=on"]),
     ("mc_replay.hip", ["-ffp-contract=off"]),
     ("cf_kernels.hip", ["-ffp-contract=off"]),
+    ("bsm_kernels.hip", ["-ffp-contract=off"]),
     ("capi.cpp", []),
     ("server.cpp", []),
     ("cf.cpp", []),

@@ -1,6 +1,6 @@
-// C ABI of the analytic pricer (include/ope_cf.h): validation, descriptors,
-// one launch of k_cf (cf_kernels.hip) per call.  The single-option entry
-// points are batches of one.
+// C ABI of the analytic pricers (include/ope_cf.h): validation, descriptors,
+// one launch per call of k_cf (cf_kernels.hip) or of k_bsm / k_iv
+// (bsm_kernels.hip).  The single-option entry points are batches of one.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -13,6 +13,10 @@
 
 namespace ope {
 hipError_t launch_cf(const CfDesc* opts, uint32_t n, double* out, CfStats* stats, hipStream_t s);
+hipError_t launch_bsm(const ope_option_contract* c, uint32_t n, double* price, ope_pricing_result* greeks,
+                      hipStream_t s);
+hipError_t launch_iv(const ope_option_contract* c, const double* mkt, uint32_t n, double* vol, uint8_t* found,
+                     hipStream_t s);
 }
 
 using namespace ope;
@@ -71,6 +75,26 @@ int make_cf_desc(const ope_option_contract& k, const ope_bates_params& bp, bool 
     return OPE_OK;
 }
 
+// The shared device buffer, grown to at least need bytes.  Called with g_cf.mu held.
+int ensure_buf(CfContext& c, size_t need) {
+    if (need <= c.cap) return OPE_OK;
+    if (c.buf) HIP_TRY(hipFree(c.buf));
+    c.buf = nullptr;
+    c.cap = 0;
+    HIP_TRY(hipMalloc(&c.buf, need));
+    c.cap = need;
+    return OPE_OK;
+}
+
+int elapsed(CfContext& c, double* kernel_ms) {
+    if (kernel_ms) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        *kernel_ms = ms;
+    }
+    return OPE_OK;
+}
+
 int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, double* kernel_ms) {
     const size_t n = descs.size();
     if (n == 0) return OPE_OK;
@@ -79,14 +103,7 @@ int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, doub
     if (int rc = cf_init(g_cf)) return rc;
     CfContext& c = g_cf;
     const size_t b_desc = sizeof(CfDesc) * n, b_out = sizeof(double) * n, b_stats = sizeof(CfStats) * n;
-    const size_t need = b_desc + b_out + b_stats;
-    if (need > c.cap) {
-        if (c.buf) HIP_TRY(hipFree(c.buf));
-        c.buf = nullptr;
-        c.cap = 0;
-        HIP_TRY(hipMalloc(&c.buf, need));
-        c.cap = need;
-    }
+    if (int rc = ensure_buf(c, b_desc + b_out + b_stats)) return rc;
     char* base = static_cast<char*>(c.buf);
     CfDesc* d_desc = reinterpret_cast<CfDesc*>(base);
     double* d_out = reinterpret_cast<double*>(base + b_desc);
@@ -107,12 +124,72 @@ int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, doub
             panels[2 * i] = st[i].panels_p1;
             panels[2 * i + 1] = st[i].panels_p2;
         }
-    if (kernel_ms) {
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-        *kernel_ms = ms;
+    return elapsed(c, kernel_ms);
+}
+
+int check_contracts(const ope_option_contract* k, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        for (double v : {k[i].spot, k[i].strike, k[i].expiry, k[i].rate, k[i].div_yield, k[i].vol})
+            if (!std::isfinite(v)) return fail(OPE_ERR_INVALID_ARGUMENT, "option %zu: non-finite argument", i);
+        if (k[i].opt_type != OPE_CALL && k[i].opt_type != OPE_PUT)
+            return fail(OPE_ERR_INVALID_ARGUMENT, "option %zu: unknown option type %d", i, k[i].opt_type);
     }
     return OPE_OK;
+}
+
+constexpr size_t kMaxLanes = 1ull << 31;  // one lane per option
+
+int run_bsm(const ope_option_contract* k, size_t n, double* price, ope_pricing_result* greeks, double* kernel_ms) {
+    if (kernel_ms) *kernel_ms = 0.0;
+    if (n == 0) return OPE_OK;
+    if (!k || (!price && !greeks)) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    if (n >= kMaxLanes) return fail(OPE_ERR_INVALID_ARGUMENT, "too many options in one call");
+    if (int rc = check_contracts(k, n)) return rc;
+    std::lock_guard<std::mutex> lk(g_cf.mu);
+    if (int rc = cf_init(g_cf)) return rc;
+    CfContext& c = g_cf;
+    const size_t b_in = sizeof(ope_option_contract) * n;
+    const size_t b_out = greeks ? sizeof(ope_pricing_result) * n : sizeof(double) * n;
+    if (int rc = ensure_buf(c, b_in + b_out)) return rc;
+    char* base = static_cast<char*>(c.buf);
+    auto* d_in = reinterpret_cast<ope_option_contract*>(base);
+    void* d_out = base + b_in;
+    HIP_TRY(hipMemcpyAsync(d_in, k, b_in, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(hipEventRecord(c.ev0, c.stream));
+    HIP_TRY(launch_bsm(d_in, (uint32_t)n, greeks ? nullptr : static_cast<double*>(d_out),
+                       greeks ? static_cast<ope_pricing_result*>(d_out) : nullptr, c.stream));
+    HIP_TRY(hipEventRecord(c.ev1, c.stream));
+    HIP_TRY(hipMemcpyAsync(greeks ? static_cast<void*>(greeks) : static_cast<void*>(price), d_out, b_out,
+                           hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    return elapsed(c, kernel_ms);
+}
+
+int run_iv(const ope_option_contract* k, const double* mkt, size_t n, double* vol, uint8_t* found, double* kernel_ms) {
+    if (kernel_ms) *kernel_ms = 0.0;
+    if (n == 0) return OPE_OK;
+    if (!k || !mkt || !vol || !found) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    if (n >= kMaxLanes) return fail(OPE_ERR_INVALID_ARGUMENT, "too many options in one call");
+    if (int rc = check_contracts(k, n)) return rc;
+    std::lock_guard<std::mutex> lk(g_cf.mu);
+    if (int rc = cf_init(g_cf)) return rc;
+    CfContext& c = g_cf;
+    const size_t b_in = sizeof(ope_option_contract) * n, b_mkt = sizeof(double) * n, b_vol = sizeof(double) * n;
+    if (int rc = ensure_buf(c, b_in + b_mkt + b_vol + n)) return rc;
+    char* base = static_cast<char*>(c.buf);
+    auto* d_in = reinterpret_cast<ope_option_contract*>(base);
+    auto* d_mkt = reinterpret_cast<double*>(base + b_in);
+    auto* d_vol = reinterpret_cast<double*>(base + b_in + b_mkt);
+    auto* d_found = reinterpret_cast<uint8_t*>(base + b_in + b_mkt + b_vol);
+    HIP_TRY(hipMemcpyAsync(d_in, k, b_in, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(hipMemcpyAsync(d_mkt, mkt, b_mkt, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(hipEventRecord(c.ev0, c.stream));
+    HIP_TRY(launch_iv(d_in, d_mkt, (uint32_t)n, d_vol, d_found, c.stream));
+    HIP_TRY(hipEventRecord(c.ev1, c.stream));
+    HIP_TRY(hipMemcpyAsync(vol, d_vol, b_vol, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipMemcpyAsync(found, d_found, n, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    return elapsed(c, kernel_ms);
 }
 
 // greeks.rs:19-82: the 14 bumped prices of one option, in this order.
@@ -267,6 +344,43 @@ int ope_batch_bates_greeks(const ope_option_contract* contracts, size_t n, const
     for (size_t i = 0; i < n; ++i)
         if (int rc = make_cf_desc(contracts[i], *params, true, d[i])) return fail(rc, "option %zu: %s", i, g_last_error.c_str());
     return run_greeks(d, out);
+}
+
+int ope_bsm_price_and_greeks(const ope_option_contract* contract, ope_pricing_result* out) {
+    if (!contract || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    return run_bsm(contract, 1, nullptr, out, nullptr);
+}
+
+int ope_bsm_price(const ope_option_contract* contract, double* out) {
+    if (!contract || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    return run_bsm(contract, 1, out, nullptr, nullptr);
+}
+
+int ope_implied_vol(const ope_option_contract* contract, double market_price, double* vol, uint8_t* found) {
+    return run_iv(contract, &market_price, 1, vol, found, nullptr);
+}
+
+int ope_batch_bsm(const ope_option_contract* contracts, size_t n, ope_pricing_result* out) {
+    return run_bsm(contracts, n, nullptr, out, nullptr);
+}
+
+int ope_batch_bsm_price(const ope_option_contract* contracts, size_t n, double* out) {
+    return run_bsm(contracts, n, out, nullptr, nullptr);
+}
+
+int ope_batch_implied_vol(const ope_option_contract* contracts, const double* market_prices, size_t n, double* vol,
+                          uint8_t* found) {
+    return run_iv(contracts, market_prices, n, vol, found, nullptr);
+}
+
+int ope_batch_bsm_ex(const ope_option_contract* contracts, size_t n, double* out_price, ope_pricing_result* out_greeks,
+                     double* kernel_ms) {
+    return run_bsm(contracts, n, out_greeks ? nullptr : out_price, out_greeks, kernel_ms);
+}
+
+int ope_batch_implied_vol_ex(const ope_option_contract* contracts, const double* market_prices, size_t n,
+                             double* vol, uint8_t* found, double* kernel_ms) {
+    return run_iv(contracts, market_prices, n, vol, found, kernel_ms);
 }
 
 int ope_cf_price_options(const ope_cf_option* options, size_t n, double* out, uint32_t* panels, double* kernel_ms) {

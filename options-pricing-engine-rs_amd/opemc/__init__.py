@@ -539,6 +539,8 @@ class Server:
 
 
 # ---------------------------------------------------------- analytic pricers
-from .cf import (CF_DTYPE, BatesParams, CfOption, OptionContract, PricingResult, batch_bates,  # noqa: E402
-                 batch_bates_greeks, batch_heston, batch_heston_greeks, bates_price, bates_price_and_greeks,
-                 cf_price_options, heston_price, heston_price_and_greeks)
+from .cf import (CF_DTYPE, CONTRACT_DTYPE, GREEKS_DTYPE, BatesParams, CfOption, OptionContract,  # noqa: E402
+                 PricingResult, batch_bates, batch_bates_greeks, batch_bsm, batch_bsm_price, batch_heston,
+                 batch_heston_greeks, batch_implied_vol, bates_price, bates_price_and_greeks, bsm_chain,
+                 bsm_price, bsm_price_and_greeks, cf_price_options, heston_price, heston_price_and_greeks,
+                 implied_vol, implied_vol_chain)

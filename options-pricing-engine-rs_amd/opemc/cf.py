@@ -70,9 +70,21 @@ def _cf_lib():
         L.ope_bates_price_and_greeks.argtypes = [d, d, d, d, d, C.POINTER(_Bates), C.c_int32, R]
         L.ope_batch_heston_greeks.argtypes = [C.POINTER(_Contract), C.c_size_t, C.POINTER(_Heston), R]
         L.ope_batch_bates_greeks.argtypes = [C.POINTER(_Contract), C.c_size_t, C.POINTER(_Bates), R]
+        K = C.POINTER(_Contract)
+        L.ope_bsm_price_and_greeks.argtypes = [K, R]
+        L.ope_bsm_price.argtypes = [K, C.POINTER(d)]
+        L.ope_implied_vol.argtypes = [K, d, C.POINTER(d), C.POINTER(C.c_uint8)]
+        L.ope_batch_bsm.argtypes = [K, C.c_size_t, R]
+        L.ope_batch_bsm_price.argtypes = [K, C.c_size_t, C.POINTER(d)]
+        L.ope_batch_implied_vol.argtypes = [K, C.POINTER(d), C.c_size_t, C.POINTER(d), C.POINTER(C.c_uint8)]
+        L.ope_batch_bsm_ex.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.POINTER(d)]
+        L.ope_batch_implied_vol_ex.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+                                               C.POINTER(d)]
         for n in ("ope_heston_price", "ope_bates_price", "ope_batch_heston", "ope_batch_bates",
                   "ope_cf_price_options", "ope_heston_price_and_greeks", "ope_bates_price_and_greeks",
-                  "ope_batch_heston_greeks", "ope_batch_bates_greeks"):
+                  "ope_batch_heston_greeks", "ope_batch_bates_greeks", "ope_bsm_price_and_greeks",
+                  "ope_bsm_price", "ope_implied_vol", "ope_batch_bsm", "ope_batch_bsm_price",
+                  "ope_batch_implied_vol", "ope_batch_bsm_ex", "ope_batch_implied_vol_ex"):
             getattr(L, n).restype = C.c_int
         L._cf_ready = True
     return L
@@ -231,3 +243,91 @@ def cf_price_options(options, with_panels=False, with_kernel_ms=False):
     if with_kernel_ms:
         res += (kms.value,)
     return res[0] if len(res) == 1 else res
+
+
+# ------------------------------------------------ Black-Scholes-Merton, IV
+
+# numpy layout of ope_option_contract (types.rs:17-25), for large chains
+CONTRACT_DTYPE = np.dtype([("spot", "f8"), ("strike", "f8"), ("expiry", "f8"), ("rate", "f8"),
+                           ("div_yield", "f8"), ("vol", "f8"), ("opt_type", "i4"), ("reserved", "i4")])
+GREEKS_DTYPE = np.dtype([(n, "f8") for n in ("price", "delta", "gamma", "vega", "theta", "rho", "vanna", "volga")])
+assert CONTRACT_DTYPE.itemsize == C.sizeof(_Contract) and GREEKS_DTYPE.itemsize == C.sizeof(_PricingResult)
+
+
+def bsm_price_and_greeks(contract: OptionContract) -> PricingResult:
+    """bsm.rs:11-35"""
+    out = _PricingResult()
+    k = contract._c()
+    _check(_cf_lib().ope_bsm_price_and_greeks(C.byref(k), C.byref(out)))
+    return PricingResult._from(out)
+
+
+def bsm_price(contract: OptionContract) -> float:
+    """bsm.rs:49-54"""
+    out = C.c_double()
+    k = contract._c()
+    _check(_cf_lib().ope_bsm_price(C.byref(k), C.byref(out)))
+    return out.value
+
+
+def implied_vol(contract: OptionContract, market_price: float) -> Optional[float]:
+    """iv.rs:14-20: None where no volatility is found."""
+    v, f = C.c_double(), C.c_uint8()
+    k = contract._c()
+    _check(_cf_lib().ope_implied_vol(C.byref(k), market_price, C.byref(v), C.byref(f)))
+    return v.value if f.value else None
+
+
+def batch_bsm(contracts: Sequence[OptionContract]):
+    """batch.rs:13-15"""
+    out = (_PricingResult * max(1, len(contracts)))()
+    _check(_cf_lib().ope_batch_bsm(_contracts(contracts), len(contracts), out))
+    return [PricingResult._from(out[i]) for i in range(len(contracts))]
+
+
+def batch_bsm_price(contracts: Sequence[OptionContract]) -> np.ndarray:
+    """batch.rs:18-20"""
+    out = np.zeros(len(contracts))
+    _check(_cf_lib().ope_batch_bsm_price(_contracts(contracts), len(contracts),
+                                         out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out
+
+
+def batch_implied_vol(contracts: Sequence[OptionContract], market_prices: Sequence[float]):
+    """batch.rs:51-56: a list with None where the solver found no volatility."""
+    n = len(contracts)
+    if len(market_prices) != n:
+        raise ValueError("one market price per contract")
+    mkt = np.ascontiguousarray(market_prices, dtype=np.float64)
+    vol = np.zeros(n)
+    found = np.zeros(n, dtype=np.uint8)
+    _check(_cf_lib().ope_batch_implied_vol(_contracts(contracts), mkt.ctypes.data_as(C.POINTER(C.c_double)), n,
+                                           vol.ctypes.data_as(C.POINTER(C.c_double)),
+                                           found.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return [float(v) if f else None for v, f in zip(vol, found)]
+
+
+def bsm_chain(contracts: np.ndarray, greeks=True):
+    """A CONTRACT_DTYPE chain in one launch: (GREEKS_DTYPE records or prices, kernel ms)."""
+    c = np.ascontiguousarray(contracts)
+    if c.dtype != CONTRACT_DTYPE:
+        raise TypeError("contracts must have dtype CONTRACT_DTYPE")
+    out = np.zeros(len(c), dtype=GREEKS_DTYPE) if greeks else np.zeros(len(c))
+    kms = C.c_double()
+    _check(_cf_lib().ope_batch_bsm_ex(c.ctypes.data, len(c), None if greeks else out.ctypes.data,
+                                      out.ctypes.data if greeks else None, C.byref(kms)))
+    return out, kms.value
+
+
+def implied_vol_chain(contracts: np.ndarray, market_prices: np.ndarray):
+    """A CONTRACT_DTYPE chain in one launch: (vols with NaN where none, found flags, kernel ms)."""
+    c = np.ascontiguousarray(contracts)
+    if c.dtype != CONTRACT_DTYPE:
+        raise TypeError("contracts must have dtype CONTRACT_DTYPE")
+    mkt = np.ascontiguousarray(market_prices, dtype=np.float64)
+    vol = np.zeros(len(c))
+    found = np.zeros(len(c), dtype=np.uint8)
+    kms = C.c_double()
+    _check(_cf_lib().ope_batch_implied_vol_ex(c.ctypes.data, mkt.ctypes.data, len(c), vol.ctypes.data,
+                                              found.ctypes.data, C.byref(kms)))
+    return vol, found, kms.value

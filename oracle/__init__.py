@@ -95,6 +95,10 @@ def lib():
     L.ora_heston_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int]
     L.ora_bates_price.restype = C.c_double
     L.ora_bates_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston)] + [C.c_double] * 3 + [C.c_int]
+    L.ora_bsm_greeks.restype = None
+    L.ora_bsm_greeks.argtypes = [C.c_double] * 6 + [C.c_int, _dp]
+    L.ora_implied_vol.restype = C.c_int
+    L.ora_implied_vol.argtypes = [C.c_double] * 5 + [C.c_int, C.c_double, C.POINTER(C.c_double)]
     L.ora_cf_greeks.restype = None
     L.ora_cf_greeks.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int] + [C.c_double] * 3 + [C.c_int, _dp]
     L.ora_cf_price_many.restype = None
@@ -187,6 +191,19 @@ def cf_greeks(s, k, t, r, q, hp, jumps=None, opt_type=CALL):
     lam, mj, sj = jumps if jumps is not None else (0.0, 0.0, 0.0)
     lib().ora_cf_greeks(s, k, t, r, q, C.byref(hp), int(jumps is not None), lam, mj, sj, opt_type, out)
     return dict(zip(GREEK_NAMES, out))
+
+
+def bsm_greeks(s, k, t, r, q, vol, opt_type=CALL):
+    """bsm.rs:11-45: price and closed-form Greeks, a dict of PricingResult's fields."""
+    out = np.zeros(8)
+    lib().ora_bsm_greeks(s, k, t, r, q, vol, opt_type, out)
+    return dict(zip(GREEK_NAMES, out))
+
+
+def implied_vol(s, k, t, r, q, opt_type, price):
+    """iv.rs:14-117; None where the reference returns None."""
+    v = C.c_double()
+    return v.value if lib().ora_implied_vol(s, k, t, r, q, opt_type, price, C.byref(v)) else None
 
 
 def cf_price_many(args, n_threads=1):

@@ -21,6 +21,9 @@
 
 typedef struct { double re, im; } cpx;
 
+double ora_ncdf(double x);
+double ora_bsm_price(double s, double k, double t, double r, double q, double vol, int opt_type);
+
 static inline cpx C(double re, double im) { cpx z = {re, im}; return z; }
 static inline cpx cadd(cpx a, cpx b) { return C(a.re + b.re, a.im + b.im); }
 static inline cpx csub(cpx a, cpx b) { return C(a.re - b.re, a.im - b.im); }
@@ -279,6 +282,82 @@ void ora_cf_greeks(double s, double k, double t, double r, double q, const ora_h
 }
 
 double ora_ncdf(double x);
+
+/* math.rs:8-13 */
+static double npdf(double x) { return 0.3989422804014327 * exp(-0.5 * x * x); }
+
+/* bsm.rs:11-45: price and closed-form Greeks.  out: price, delta, gamma,
+ * vega, theta, rho, vanna, volga. */
+void ora_bsm_greeks(double s, double k, double t, double r, double q, double v, int opt_type, double *out) {
+    double vt = v * sqrt(t);
+    double d1 = (log(s / k) + (r - q + 0.5 * v * v) * t) / vt;
+    double d2 = d1 - vt;
+    double phi = opt_type == ORA_CALL ? 1.0 : -1.0;
+    double nd1 = ora_ncdf(phi * d1), nd2 = ora_ncdf(phi * d2), npd1 = npdf(d1);
+    double eq = exp(-q * t), er = exp(-r * t);
+    double seq = s * eq, ker = k * er, sqt = sqrt(t);
+    out[0] = phi * (seq * nd1 - ker * nd2);
+    out[1] = phi * eq * nd1;
+    out[2] = eq * npd1 / (s * v * sqt);
+    out[3] = seq * npd1 * sqt;
+    out[4] = -seq * npd1 * (v * sqt) / (2.0 * t) - phi * (q * seq * nd1 - r * ker * nd2);
+    out[5] = phi * k * t * er * nd2;
+    out[6] = -eq * npd1 * d2 / v;
+    out[7] = out[3] * d1 * d2 / v;
+}
+
+/* iv.rs:14-117: bounds check, Brenner-Subrahmanyam guess (or bisection),
+ * Halley.  Returns 1 and *vol when a volatility is found, else 0. */
+static double bsm_at(double s, double k, double t, double r, double q, double v, int opt) {
+    return ora_bsm_price(s, k, t, r, q, v, opt);
+}
+
+static int iv_bisect(double s, double k, double t, double r, double q, int opt, double target, double *vol) {
+    double lo = 1e-4, hi = 10.0;
+    double flo = bsm_at(s, k, t, r, q, lo, opt) - target, fhi = bsm_at(s, k, t, r, q, hi, opt) - target;
+    if (flo * fhi > 0.0) return 0;
+    for (int i = 0; i < 60; ++i) {
+        double mid = 0.5 * (lo + hi);
+        if (bsm_at(s, k, t, r, q, mid, opt) - target < 0.0) lo = mid; else hi = mid;
+        if (hi - lo < 1e-9) { *vol = mid; return 1; }
+    }
+    *vol = 0.5 * (lo + hi);
+    return 1;
+}
+
+int ora_implied_vol(double s, double k, double t, double r, double q, int opt, double price, double *vol) {
+    /* price_in_bounds (iv.rs:22-32) */
+    if (price <= 0.0) return 0;
+    double er = exp(-r * t), eq = exp(-q * t);
+    double intrinsic = opt == ORA_CALL ? fmax(s * eq - k * er, 0.0) : fmax(k * er - s * eq, 0.0);
+    if (!(price >= intrinsic)) return 0;
+    /* initial_guess (iv.rs:37-50) */
+    double fwd = s * eq / er;
+    double bs = price * sqrt(2.0 * 3.141592653589793 / t) / (fwd * er);
+    double x = log(fwd / k);
+    double v = bs / fmax(exp(-0.5 * x * x), 0.01);
+    v = v < 0.001 ? 0.001 : (v > 10.0 ? 10.0 : v);
+    if (!(fabs(bsm_at(s, k, t, r, q, v, opt) - price) < 0.2 * price)) {
+        if (!iv_bisect(s, k, t, r, q, opt, price, &v)) return 0;
+    }
+    /* halley_solve (iv.rs:68-90) with vega_and_volga (93-103) */
+    for (int it = 0; it < 10; ++it) {
+        double err = bsm_at(s, k, t, r, q, v, opt) - price;
+        if (fabs(err) < 1e-10) { *vol = v; return 1; }
+        double vt = v * sqrt(t);
+        double d1 = (log(s / k) + (r - q + 0.5 * v * v) * t) / vt;
+        double d2 = d1 - vt;
+        double vega = s * eq * npdf(d1) * sqrt(t);
+        double volga = vega * d1 * d2 / v;
+        if (fabs(vega) < 1e-14) return 0;
+        double denom = 1.0 - err * volga / (2.0 * vega * vega);
+        denom = denom < 0.5 ? 0.5 : (denom > 2.0 ? 2.0 : denom);
+        v -= err / (vega * denom);
+        if (v <= 0.0) v = 1e-8;
+    }
+    if (fabs(bsm_at(s, k, t, r, q, v, opt) - price) < 1e-6) { *vol = v; return 1; }
+    return 0;
+}
 
 /* bsm.rs:7-12, 55-60 */
 double ora_bsm_price(double s, double k, double t, double r, double q, double vol, int opt_type) {

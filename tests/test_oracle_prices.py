@@ -235,3 +235,47 @@ def test_cf_greeks_reference_signs(ora):
     assert -1.0 < p["delta"] < 0.0 and p["gamma"] > 0
     b = ora.cf_greeks(100.0, 100.0, 1.0, 0.05, 0.0, hp, jumps=(0.0, 0.0, 1e-8))
     assert abs(b["delta"] - c["delta"]) < 0.01 and abs(b["vega"] - c["vega"]) < 1.0
+
+
+@pytest.mark.parametrize("k,t,q,ot", [(100.0, 1.0, 0.0, 0), (80.0, 0.25, 0.02, 1), (130.0, 2.0, 0.01, 0)])
+def test_bsm_greeks_are_the_derivatives_of_the_price(ora, k, t, q, ot):
+    # bsm.rs:11-45 against central differences of bsm_price (bsm.rs:49-54)
+    s, r, v = 100.0, 0.03, 0.25
+    g = ora.bsm_greeks(s, k, t, r, q, v, ot)
+    P = lambda s=s, t=t, r=r, v=v: ora.bsm_price(s, k, t, r, q, v, ot)
+    h = 1e-4
+    assert g["price"] == P()
+    assert abs(g["delta"] - (P(s=s + h) - P(s=s - h)) / (2 * h)) < 1e-7
+    assert abs(g["gamma"] - (P(s=s + 1e-2) - 2 * P() + P(s=s - 1e-2)) / 1e-4) < 1e-6
+    assert abs(g["vega"] - (P(v=v + h) - P(v=v - h)) / (2 * h)) < 1e-7 * max(1.0, g["vega"])
+    # Reference quirk (bsm.rs:38-45): theta's carry term -phi (q S e^{-qT} N(phi d1)
+    # - r K e^{-rT} N(phi d2)) has the opposite sign of -dP/dT's, so theta is
+    # -dP/dT - 2 x carry; the vega-like term S e^{-qT} n(d1) v / (2 sqrt T) is right.
+    # Checked as: theta + (-dP/dT) = -2 x that term.
+    from scipy.stats import norm
+    d1 = (math.log(s / k) + (r - q + 0.5 * v * v) * t) / (v * math.sqrt(t))
+    a = s * math.exp(-q * t) * norm.pdf(d1) * v / (2.0 * math.sqrt(t))
+    minus_dpdt = -(P(t=t + h) - P(t=t - h)) / (2 * h)
+    assert abs(g["theta"] + minus_dpdt + 2.0 * a) < 1e-6 * max(1.0, abs(g["theta"]))
+    assert abs(g["rho"] - (P(r=r + h) - P(r=r - h)) / (2 * h)) < 1e-6 * max(1.0, abs(g["rho"]))
+    dv = lambda v: (P(s=s + 1e-3, v=v) - P(s=s - 1e-3, v=v)) / 2e-3
+    assert abs(g["vanna"] - (dv(v + 1e-3) - dv(v - 1e-3)) / 2e-3) < 1e-4
+    assert abs(g["volga"] - (P(v=v + 1e-3) - 2 * P() + P(v=v - 1e-3)) / 1e-6) < 1e-3 * max(1.0, abs(g["volga"]))
+
+
+@pytest.mark.parametrize("vol,ot", [(0.2, 0), (0.2, 1), (0.3, 0), (0.05, 0)])
+def test_implied_vol_round_trips(ora, vol, ot):
+    # iv.rs:120-126 (S=K=100, T=0.5, r=0.03)
+    px = ora.bsm_price(100.0, 100.0, 0.5, 0.03, 0.0, vol, ot)
+    assert abs(ora.implied_vol(100.0, 100.0, 0.5, 0.03, 0.0, ot, px) - vol) < 1e-7
+
+
+def test_implied_vol_rejects_and_wings(ora):
+    # iv.rs:128-131: a negative price has no volatility; below intrinsic neither
+    assert ora.implied_vol(100.0, 100.0, 0.5, 0.03, 0.0, 0, -1.0) is None
+    assert ora.implied_vol(100.0, 60.0, 0.5, 0.03, 0.0, 0, 39.0) is None
+    # deep wings, where the initial guess falls back to bisection
+    for k, vol in ((40.0, 0.6), (250.0, 0.45), (100.0, 2.5)):
+        px = ora.bsm_price(100.0, k, 1.0, 0.03, 0.0, vol, 0)
+        got = ora.implied_vol(100.0, k, 1.0, 0.03, 0.0, 0, px)
+        assert got is not None and abs(got - vol) < 1e-6, (k, vol, got)
