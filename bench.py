@@ -20,10 +20,11 @@ One step is one pass of the hot path over one batch of synthetic input:
 * serving: the persistent-kernel server under Poisson arrivals at 2000 req/s,
   each request run to a standard error of 1e-3 x price; p50/p99 latency.
 * analytic_cf: the GPU characteristic-function pricer on 32 parameter sets x
-  the 41 x 8 surface, Heston and Bates (20992 options per call).
+  the 41 x 8 surface, Heston and Bates (20992 options per call), and the BSM
+  price-and-Greeks and implied-volatility chains (4M and 1M options).
 
-The default run also measures bates_surface, uneven_batch, analytic_cf and
-serving and carries them in the same JSON line.
+The default run also measures bates_surface, uneven_batch, analytic_cf,
+bsm_iv and serving and carries them in the same JSON line.
 
 The value is FP32-path mode path-steps/s (both antithetic siblings count);
 FP64 mode runs the same workload on identical Philox streams and the line
@@ -262,6 +263,69 @@ def analytic_cf(ope, wm, steps=5, with_cpu=True):
     return out
 
 
+def bsm_iv(ope, wm, steps=5, with_cpu=True, n_bsm=1 << 22, n_iv=1 << 20):
+    """BSM price-and-Greeks chain (HBM-bound: a 56-byte contract in, a 64-byte
+    result out per option) and an implied-volatility chain on the GPU, with
+    the oracle's restatements of bsm.rs / iv.rs on the host cores beside them."""
+    import numpy as np
+
+    rng = np.random.default_rng(21)
+    c = np.zeros(n_bsm, dtype=ope.CONTRACT_DTYPE)
+    c["spot"] = 100.0
+    c["strike"] = rng.uniform(50.0, 200.0, n_bsm)
+    c["expiry"] = rng.uniform(0.05, 3.0, n_bsm)
+    c["rate"] = 0.03
+    c["div_yield"] = rng.uniform(0.0, 0.03, n_bsm)
+    c["vol"] = rng.uniform(0.1, 0.8, n_bsm)
+    c["opt_type"] = rng.integers(0, 2, n_bsm)
+    g, _ = ope.bsm_chain(c)  # warm-up
+    kms, t0 = [], time.perf_counter()
+    for _ in range(steps):
+        g, k = ope.bsm_chain(c)
+        kms.append(k)
+    wall = (time.perf_counter() - t0) / steps
+    k_bsm = sum(kms) / len(kms)
+    nbytes = n_bsm * (ope.CONTRACT_DTYPE.itemsize + ope.GREEKS_DTYPE.itemsize)
+    gbs = nbytes / (k_bsm * 1e-3) / 1e9
+    ci, prices = c[:n_iv], g["price"][:n_iv]
+    ope.implied_vol_chain(ci, prices)
+    kms, t0 = [], time.perf_counter()
+    for _ in range(steps):
+        vol, found, k = ope.implied_vol_chain(ci, prices)
+        kms.append(k)
+    wall_iv = (time.perf_counter() - t0) / steps
+    k_iv = sum(kms) / len(kms)
+    out = {"workload": (f"{n_bsm} BSM contracts (S=100, K ~ U[50, 200], T ~ U[0.05, 3], r=0.03, q ~ U[0, 0.03], "
+                        f"vol ~ U[0.1, 0.8], calls and puts, numpy seed 21): price and 7 Greeks; implied vol of "
+                        f"the first {n_iv} from their prices"),
+           "bsm_options_per_s": n_bsm / (k_bsm * 1e-3), "bsm_kernel_ms": k_bsm, "bsm_ms_per_call": wall * 1e3,
+           "bsm_roofline": {"bound": "hbm", "achieved": gbs, "peak": wm.PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": gbs / wm.PEAK_HBM_GBS, "traffic": None},
+           "iv_options_per_s": n_iv / (k_iv * 1e-3), "iv_kernel_ms": k_iv, "iv_ms_per_call": wall_iv * 1e3,
+           "iv_found": int(found.sum()), "dtype": "f64"}
+    print(f"# bsm: {n_bsm} options kernel {k_bsm:.3f} ms ({gbs:.0f} GB/s); iv: {n_iv} kernel {k_iv:.3f} ms",
+          file=sys.stderr, flush=True)
+    if with_cpu:
+        import oracle
+
+        cores = max(1, min(16, os.cpu_count() or 1))
+        m = 1 << 18
+        a = np.column_stack([c[f][:m] for f in ("spot", "strike", "expiry", "rate", "div_yield", "vol")]
+                            + [c["opt_type"][:m].astype(float)])
+        t0 = time.perf_counter()
+        ref = oracle.bsm_greeks_many(a, n_threads=cores)
+        t_bsm = time.perf_counter() - t0
+        b = np.column_stack([a[:, :5], a[:, 6], ref[:, 0]])
+        t0 = time.perf_counter()
+        oracle.implied_vol_many(b, n_threads=cores)
+        t_iv = time.perf_counter() - t0
+        out["max_rel_diff_vs_oracle"] = float(np.max(np.abs(g["price"][:m] - ref[:, 0]) / np.maximum(1.0, ref[:, 0])))
+        out["cpu_baseline"] = {"bsm_options_per_s": m / t_bsm, "iv_options_per_s": m / t_iv, "cores": cores,
+                               "kind": "port", "sample": f"the first {m} contracts, C restatement of bsm.rs / iv.rs, "
+                                                         f"{cores} threads"}
+    return out
+
+
 def serving(ope, seconds):
     """The serving configuration (Poisson arrivals at 2000 req/s) in both
     precisions, plus the same requests all at once (saturated throughput)."""
@@ -372,7 +436,8 @@ def main():
         print(json.dumps({"serving": serving(ope, args.serve_seconds)}))
         return
     if args.workload == "analytic_cf":
-        print(json.dumps({"analytic_cf": analytic_cf(ope, wm, steps=args.steps, with_cpu=not args.no_cpu)}))
+        print(json.dumps({"analytic_cf": analytic_cf(ope, wm, steps=args.steps, with_cpu=not args.no_cpu),
+                          "bsm_iv": bsm_iv(ope, wm, steps=args.steps, with_cpu=not args.no_cpu)}))
         return
     desc, conf, run, path_steps, flops = make_workload(ope, wm, args.workload)
 
@@ -422,6 +487,7 @@ def main():
         for extra in ("bates_surface", "uneven_batch"):
             line[extra] = secondary(ope, wm, extra)
         line["analytic_cf"] = analytic_cf(ope, wm, with_cpu=not args.no_cpu)
+        line["bsm_iv"] = bsm_iv(ope, wm, with_cpu=not args.no_cpu)
     if not args.no_serve:
         line["serving"] = serving(ope, args.serve_seconds)
     if not args.no_cpu:

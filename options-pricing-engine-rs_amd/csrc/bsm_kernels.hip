@@ -4,9 +4,10 @@
 //
 // The price-and-Greeks kernel reads a 56-byte contract and writes a 64-byte
 // result per option with a few dozen FP64 operations in between: it is bound
-// by HBM bandwidth.  The implied-volatility kernel re-prices each option
-// 3-15 times (guess check, Halley steps, or the bisection fallback) and is
-// bound by FP64 issue; its lanes diverge only in their iteration counts.
+// by HBM bandwidth, and stages its results through LDS.  The
+// implied-volatility kernel re-prices each option 3-15 times (guess check,
+// Halley steps, or the bisection fallback) and is bound by FP64 issue; its
+// lanes diverge only in their iteration counts.
 #include <hip/hip_runtime.h>
 
 #include "../../include/ope_cf.h"
@@ -48,16 +49,41 @@ __device__ ope_pricing_result bsm_greeks(double s, double k, double t, double r,
     return g;
 }
 
-__global__ __launch_bounds__(256) void k_bsm(const ope_option_contract* __restrict__ c, uint32_t n,
-                                             double* __restrict__ price, ope_pricing_result* __restrict__ greeks) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    const ope_option_contract o = c[i];
-    const double phi = o.opt_type == OPE_CALL ? 1.0 : -1.0;
-    if (greeks)
-        greeks[i] = bsm_greeks(o.spot, o.strike, o.expiry, o.rate, o.div_yield, o.vol, phi);
-    else
-        price[i] = bsm_price(o.spot, o.strike, o.expiry, o.rate, o.div_yield, o.vol, phi);
+constexpr int kBsmBlock = 256;
+constexpr int kResultWords = (int)(sizeof(ope_pricing_result) / 8);     // 8
+static_assert(sizeof(ope_option_contract) == 56 && sizeof(ope_pricing_result) == 64, "record layouts");
+
+// Each thread reads its contract directly (the 56-byte records of a wave
+// are contiguous, so the L2 sees whole lines), but the 64-byte results go
+// through LDS and leave as one contiguous run of 8-byte words per block
+// (consecutive lanes, consecutive addresses): per-lane 64-byte-strided stores
+// ran at 4.1 TB/s, the staged ones at 5.3 TB/s, and staging the loads as
+// well was slower (3.7 TB/s).  The staging row is padded to 9 words to
+// spread the per-thread writes over the LDS banks.
+__global__ __launch_bounds__(kBsmBlock) void k_bsm(const ope_option_contract* __restrict__ c, uint32_t n,
+                                                   double* __restrict__ price, ope_pricing_result* __restrict__ greeks) {
+    __shared__ double s_out[kBsmBlock * (kResultWords + 1)];
+    const uint32_t base = blockIdx.x * (uint32_t)kBsmBlock;
+    const uint32_t cnt = min((uint32_t)kBsmBlock, n - base);
+    const uint32_t tid = threadIdx.x;
+    if (tid < cnt) {
+        const ope_option_contract o = c[base + tid];
+        const double phi = o.opt_type == OPE_CALL ? 1.0 : -1.0;
+        if (greeks) {
+            const ope_pricing_result g = bsm_greeks(o.spot, o.strike, o.expiry, o.rate, o.div_yield, o.vol, phi);
+            const double* gv = reinterpret_cast<const double*>(&g);
+#pragma unroll
+            for (int f = 0; f < kResultWords; ++f) s_out[tid * (kResultWords + 1) + f] = gv[f];
+        } else {
+            price[base + tid] = bsm_price(o.spot, o.strike, o.expiry, o.rate, o.div_yield, o.vol, phi);
+        }
+    }
+    if (greeks) {
+        __syncthreads();
+        double* dst = reinterpret_cast<double*>(greeks + base);
+        for (uint32_t j = tid; j < cnt * kResultWords; j += kBsmBlock)
+            dst[j] = s_out[(j / kResultWords) * (kResultWords + 1) + (j % kResultWords)];
+    }
 }
 
 // iv.rs:14-117: bounds, Brenner-Subrahmanyam guess checked against 20 % of
@@ -132,7 +158,7 @@ __global__ __launch_bounds__(256) void k_iv(const ope_option_contract* __restric
 hipError_t launch_bsm(const ope_option_contract* c, uint32_t n, double* price, ope_pricing_result* greeks,
                       hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bsm, dim3((n + 255) / 256), dim3(256), 0, s, c, n, price, greeks);
+    hipLaunchKernelGGL(k_bsm, dim3((n + kBsmBlock - 1) / kBsmBlock), dim3(kBsmBlock), 0, s, c, n, price, greeks);
     return hipGetLastError();
 }
 

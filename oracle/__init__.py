@@ -99,6 +99,10 @@ def lib():
     L.ora_bsm_greeks.argtypes = [C.c_double] * 6 + [C.c_int, _dp]
     L.ora_implied_vol.restype = C.c_int
     L.ora_implied_vol.argtypes = [C.c_double] * 5 + [C.c_int, C.c_double, C.POINTER(C.c_double)]
+    L.ora_bsm_greeks_many.restype = None
+    L.ora_bsm_greeks_many.argtypes = [_dp, C.c_int64, _dp]
+    L.ora_implied_vol_many.restype = None
+    L.ora_implied_vol_many.argtypes = [_dp, C.c_int64, _dp]
     L.ora_cf_greeks.restype = None
     L.ora_cf_greeks.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int] + [C.c_double] * 3 + [C.c_int, _dp]
     L.ora_cf_price_many.restype = None
@@ -204,6 +208,33 @@ def implied_vol(s, k, t, r, q, opt_type, price):
     """iv.rs:14-117; None where the reference returns None."""
     v = C.c_double()
     return v.value if lib().ora_implied_vol(s, k, t, r, q, opt_type, price, C.byref(v)) else None
+
+
+def _threaded_rows(fn, args, width_in, width_out, n_threads):
+    a = np.ascontiguousarray(args, dtype=np.float64).reshape(-1, width_in)
+    n = a.shape[0]
+    out = np.zeros((n, width_out)) if width_out > 1 else np.zeros(n)
+    bounds = np.linspace(0, n, max(1, n_threads) + 1).astype(int)
+
+    def run(k):
+        lo, hi = int(bounds[k]), int(bounds[k + 1])
+        if hi > lo:
+            fn(a[lo:hi], hi - lo, out[lo:hi])
+
+    import concurrent.futures
+    with concurrent.futures.ThreadPoolExecutor(max(1, n_threads)) as ex:
+        list(ex.map(run, range(len(bounds) - 1)))
+    return out
+
+
+def bsm_greeks_many(args, n_threads=1):
+    """Rows (spot, strike, expiry, rate, div_yield, vol, opt_type) -> n x 8 price and Greeks."""
+    return _threaded_rows(lib().ora_bsm_greeks_many, args, 7, 8, n_threads)
+
+
+def implied_vol_many(args, n_threads=1):
+    """Rows (spot, strike, expiry, rate, div_yield, opt_type, price) -> vols (NaN: none)."""
+    return _threaded_rows(lib().ora_implied_vol_many, args, 7, 1, n_threads)
 
 
 def cf_price_many(args, n_threads=1):

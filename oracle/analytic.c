@@ -359,6 +359,25 @@ int ora_implied_vol(double s, double k, double t, double r, double q, int opt, d
     return 0;
 }
 
+/* Chains of the two above, the loops batch.rs:13-15 and 51-56 run per
+ * thread.  args rows: spot, strike, expiry, rate, div_yield, vol, opt_type
+ * (Greeks) or spot, strike, expiry, rate, div_yield, opt_type, price (IV);
+ * an implied volatility that is not found is NaN. */
+void ora_bsm_greeks_many(const double *args, int64_t n, double *out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double *a = args + 7 * i;
+        ora_bsm_greeks(a[0], a[1], a[2], a[3], a[4], a[5], (int)a[6], out + 8 * i);
+    }
+}
+
+void ora_implied_vol_many(const double *args, int64_t n, double *vol) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double *a = args + 7 * i;
+        double v;
+        vol[i] = ora_implied_vol(a[0], a[1], a[2], a[3], a[4], (int)a[5], a[6], &v) ? v : NAN;
+    }
+}
+
 /* bsm.rs:7-12, 55-60 */
 double ora_bsm_price(double s, double k, double t, double r, double q, double vol, int opt_type) {
     double vt = vol * sqrt(t);

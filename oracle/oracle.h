@@ -24,6 +24,8 @@
  *   ora_bates_price           bates.rs:29-83
  *   ora_cf_price_many         a chain of the two above (batch.rs:23-42)
  *   ora_cf_greeks             greeks.rs:19-82 bump-and-reprice over the two above
+ *   ora_bsm_greeks            bsm.rs:11-45 (and ora_bsm_greeks_many, a chain)
+ *   ora_implied_vol           iv.rs:14-117 (and ora_implied_vol_many, a chain)
  *   ora_bsm_price             bsm.rs:7-12, 55-60
  *   ora_merton_price          textbook Merton (1976) series (no reference counterpart)
  */
