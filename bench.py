@@ -242,7 +242,7 @@ def analytic_cf(ope, wm, steps=5, with_cpu=True):
            "kernel_ms": kernel_ms, "kernel_options_per_s": n / (kernel_ms * 1e-3),
            "mean_panels_per_integral": float(panels.mean()),
            "roofline": {"bound": "valu", "achieved": achieved, "peak": wm.PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
-                        "frac": achieved / wm.PEAK_TFLOPS["f64"]},
+                        "frac": achieved / wm.PEAK_TFLOPS["f64"], "traffic": load_traffic("analytic_cf")},
            "dtype": "f64"}
     print(f"# analytic_cf: {n} options {wall * 1e3:.2f} ms/call, kernel {kernel_ms:.2f} ms",
           file=sys.stderr, flush=True)
@@ -300,7 +300,8 @@ def bsm_iv(ope, wm, steps=5, with_cpu=True, n_bsm=1 << 22, n_iv=1 << 20):
                         f"the first {n_iv} from their prices"),
            "bsm_options_per_s": n_bsm / (k_bsm * 1e-3), "bsm_kernel_ms": k_bsm, "bsm_ms_per_call": wall * 1e3,
            "bsm_roofline": {"bound": "hbm", "achieved": gbs, "peak": wm.PEAK_HBM_GBS, "unit": "GB/s",
-                            "frac": gbs / wm.PEAK_HBM_GBS, "traffic": None},
+                            "frac": gbs / wm.PEAK_HBM_GBS,
+                            "traffic": load_traffic("bsm_chain_greeks") if n_bsm == 1 << 22 else None},
            "iv_options_per_s": n_iv / (k_iv * 1e-3), "iv_kernel_ms": k_iv, "iv_ms_per_call": wall_iv * 1e3,
            "iv_found": int(found.sum()), "dtype": "f64"}
     print(f"# bsm: {n_bsm} options kernel {k_bsm:.3f} ms ({gbs:.0f} GB/s); iv: {n_iv} kernel {k_iv:.3f} ms",
