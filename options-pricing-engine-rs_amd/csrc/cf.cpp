@@ -3,10 +3,13 @@
 // (bsm_kernels.hip).  The single-option entry points are batches of one.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <mutex>
 #include <vector>
 
+#include "../../include/ope_calib.h"
 #include "../../include/ope_cf.h"
 #include "cf_desc.h"
 #include "engine.h"
@@ -250,6 +253,83 @@ int run_greeks(const std::vector<CfDesc>& base, ope_pricing_result* out) {
     return OPE_OK;
 }
 
+// calibration.rs:123-126 and 162-171: bump sizes and bounds per parameter,
+// in CalibModel::to_vec order.
+constexpr double kCalBump[8] = {1e-4, 1e-3, 1e-4, 1e-3, 1e-4, 1e-4, 1e-4, 1e-4};
+constexpr double kCalLo[8] = {1e-8, 1e-6, 1e-8, 1e-6, -0.9999, 0.0, -2.0, 1e-6};
+constexpr double kCalHi[8] = {5.0, 50.0, 5.0, 10.0, 0.9999, 10.0, 2.0, 2.0};
+
+// Residuals (and, with jac, the Jacobian) of one calibration evaluation:
+// every price in one k_cf launch, every implied volatility in one k_iv launch.
+int calib_eval(const ope_calib_quote* quotes, size_t n, int32_t model, const double* params, double* res,
+               double* jac) {
+    if (model != OPE_CF_HESTON && model != OPE_CF_BATES)
+        return fail(OPE_ERR_INVALID_ARGUMENT, "unknown model %d", model);
+    const int dim = model == OPE_CF_BATES ? 8 : 5;
+    for (int c = 0; c < dim; ++c)
+        if (!std::isfinite(params[c])) return fail(OPE_ERR_INVALID_ARGUMENT, "non-finite parameter %d", c);
+    // parameter vectors: 0 = base (residuals), then (up, down) per column
+    std::vector<std::array<double, 8>> vecs;
+    std::vector<double> dist;
+    std::array<double, 8> base{};
+    for (int c = 0; c < dim; ++c) base[c] = params[c];
+    if (res) vecs.push_back(base);
+    std::vector<int> col_of;  // for each pair of vectors, its column
+    if (jac)
+        for (int c = 0; c < dim; ++c) {
+            std::array<double, 8> vu = base, vd = base;
+            vu[c] = std::fmin(base[c] + kCalBump[c], kCalHi[c]);
+            vd[c] = std::fmax(base[c] - kCalBump[c], kCalLo[c]);
+            const double h = vu[c] - vd[c];
+            if (h <= 0.0) continue;  // pinned against both bounds: zero column
+            vecs.push_back(vu);
+            vecs.push_back(vd);
+            dist.push_back(h);
+            col_of.push_back(c);
+        }
+    const size_t nv = vecs.size();
+    if (n > kMaxOptions / std::max<size_t>(nv, 1)) return fail(OPE_ERR_INVALID_ARGUMENT, "too many quotes");
+    std::vector<CfDesc> d(nv * n);
+    std::vector<ope_option_contract> k(nv * n);
+    std::vector<double> iv_mkt(n);
+    for (size_t i = 0; i < n; ++i) {
+        iv_mkt[i] = quotes[i].iv_market;
+        if (!std::isfinite(quotes[i].iv_market) || !std::isfinite(quotes[i].weight))
+            return fail(OPE_ERR_INVALID_ARGUMENT, "quote %zu: non-finite iv_market or weight", i);
+    }
+    for (size_t v = 0; v < nv; ++v) {
+        ope_bates_params bp{};
+        bp.heston = ope_heston_params{vecs[v][0], vecs[v][1], vecs[v][2], vecs[v][3], vecs[v][4]};
+        bp.lambda = vecs[v][5];
+        bp.mu_j = vecs[v][6];
+        bp.sigma_j = vecs[v][7];
+        for (size_t i = 0; i < n; ++i) {
+            if (int rc = make_cf_desc(quotes[i].contract, bp, model == OPE_CF_BATES, d[v * n + i]))
+                return fail(rc, "quote %zu: %s", i, g_last_error.c_str());
+            k[v * n + i] = quotes[i].contract;
+        }
+    }
+    std::vector<double> px(nv * n), iv(nv * n);
+    std::vector<uint8_t> found(nv * n);
+    if (int rc = run_cf(d, px.data(), nullptr, nullptr)) return rc;
+    if (int rc = run_iv(k.data(), px.data(), nv * n, iv.data(), found.data(), nullptr)) return rc;
+    // calibration.rs:573-581: iv - iv_market, or 0 where no volatility is found
+    auto r = [&](size_t v, size_t i) { return found[v * n + i] ? iv[v * n + i] - iv_mkt[i] : 0.0; };
+    size_t v0 = 0;
+    if (res) {
+        for (size_t i = 0; i < n; ++i) res[i] = r(0, i);
+        v0 = 1;
+    }
+    if (jac) {
+        for (size_t i = 0; i < n * (size_t)dim; ++i) jac[i] = 0.0;
+        for (size_t p = 0; p < col_of.size(); ++p) {
+            const size_t vu = v0 + 2 * p, vd = vu + 1;
+            for (size_t i = 0; i < n; ++i) jac[i * dim + col_of[p]] = (r(vu, i) - r(vd, i)) / dist[p];
+        }
+    }
+    return OPE_OK;
+}
+
 ope_option_contract contract(double spot, double strike, double expiry, double rate, double div_yield,
                              int32_t opt_type) {
     ope_option_contract k{};
@@ -381,6 +461,37 @@ int ope_batch_bsm_ex(const ope_option_contract* contracts, size_t n, double* out
 int ope_batch_implied_vol_ex(const ope_option_contract* contracts, const double* market_prices, size_t n,
                              double* vol, uint8_t* found, double* kernel_ms) {
     return run_iv(contracts, market_prices, n, vol, found, kernel_ms);
+}
+
+int ope_calib_residuals(const ope_calib_quote* quotes, size_t n, int32_t model, const double* params,
+                        double* residuals) {
+    if (n == 0) return OPE_OK;
+    if (!quotes || !params || !residuals) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    return calib_eval(quotes, n, model, params, residuals, nullptr);
+}
+
+int ope_calib_jacobian(const ope_calib_quote* quotes, size_t n, int32_t model, const double* params,
+                       double* residuals, double* jacobian, double* jtj, double* grad) {
+    if (n == 0) return OPE_OK;
+    if (!quotes || !params || !jacobian) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    if (grad && !residuals) return fail(OPE_ERR_INVALID_ARGUMENT, "grad needs residuals");
+    if (int rc = calib_eval(quotes, n, model, params, residuals, jacobian)) return rc;
+    const int dim = model == OPE_CF_BATES ? 8 : 5;
+    // calibration.rs:655-667
+    if (jtj)
+        for (int a = 0; a < dim * dim; ++a) jtj[a] = 0.0;
+    if (grad)
+        for (int a = 0; a < dim; ++a) grad[a] = 0.0;
+    for (size_t row = 0; row < n; ++row) {
+        const double w2 = quotes[row].weight * quotes[row].weight;
+        const double* j = jacobian + row * dim;
+        for (int c1 = 0; c1 < dim; ++c1) {
+            if (grad) grad[c1] += w2 * j[c1] * residuals[row];
+            if (jtj)
+                for (int c2 = 0; c2 < dim; ++c2) jtj[c1 * dim + c2] += w2 * j[c1] * j[c2];
+        }
+    }
+    return OPE_OK;
 }
 
 int ope_cf_price_options(const ope_cf_option* options, size_t n, double* out, uint32_t* panels, double* kernel_ms) {

@@ -543,4 +543,4 @@ from .cf import (CF_DTYPE, CONTRACT_DTYPE, GREEKS_DTYPE, BatesParams, CfOption, 
                  PricingResult, batch_bates, batch_bates_greeks, batch_bsm, batch_bsm_price, batch_heston,
                  batch_heston_greeks, batch_implied_vol, bates_price, bates_price_and_greeks, bsm_chain,
                  bsm_price, bsm_price_and_greeks, cf_price_options, heston_price, heston_price_and_greeks,
-                 implied_vol, implied_vol_chain)
+                 implied_vol, implied_vol_chain, CALIB_DTYPE, calib_jacobian, calib_residuals)

@@ -331,3 +331,56 @@ def implied_vol_chain(contracts: np.ndarray, market_prices: np.ndarray):
     _check(_cf_lib().ope_batch_implied_vol_ex(c.ctypes.data, mkt.ctypes.data, len(c), vol.ctypes.data,
                                               found.ctypes.data, C.byref(kms)))
     return vol, found, kms.value
+
+
+# ------------------------------------------------------------ calibration
+
+CALIB_DTYPE = np.dtype([("contract", CONTRACT_DTYPE), ("iv_market", "f8"), ("weight", "f8")])
+
+
+def _calib_lib():
+    L = _cf_lib()
+    if not getattr(L, "_calib_ready", False):
+        P = C.c_void_p
+        L.ope_calib_residuals.argtypes = [P, C.c_size_t, C.c_int32, P, P]
+        L.ope_calib_jacobian.argtypes = [P, C.c_size_t, C.c_int32, P, P, P, P, P]
+        L.ope_calib_residuals.restype = C.c_int
+        L.ope_calib_jacobian.restype = C.c_int
+        L._calib_ready = True
+    return L
+
+
+def _calib_params(params):
+    if isinstance(params, BatesParams):
+        h = params.heston
+        return CF_BATES, np.array([h.v0, h.kappa, h.theta, h.sigma, h.rho, params.lambda_, params.mu_j,
+                                   params.sigma_j])
+    if isinstance(params, HestonParams):
+        return CF_HESTON, np.array([params.v0, params.kappa, params.theta, params.sigma, params.rho])
+    raise TypeError("params must be HestonParams or BatesParams")
+
+
+def calib_residuals(quotes: np.ndarray, params) -> np.ndarray:
+    """calibration.rs:573-587 on the GPU: iv(model price) - iv_market per quote
+    (0 where no volatility is found).  quotes: CALIB_DTYPE records."""
+    q = np.ascontiguousarray(quotes)
+    if q.dtype != CALIB_DTYPE:
+        raise TypeError("quotes must have dtype CALIB_DTYPE")
+    model, p = _calib_params(params)
+    res = np.zeros(len(q))
+    _check(_calib_lib().ope_calib_residuals(q.ctypes.data, len(q), model, p.ctypes.data, res.ctypes.data))
+    return res
+
+
+def calib_jacobian(quotes: np.ndarray, params):
+    """calibration.rs:613-667 on the GPU: (residuals, Jacobian n x dim, J'J, J'r)."""
+    q = np.ascontiguousarray(quotes)
+    if q.dtype != CALIB_DTYPE:
+        raise TypeError("quotes must have dtype CALIB_DTYPE")
+    model, p = _calib_params(params)
+    n, dim = len(q), len(p)
+    res, jac = np.zeros(n), np.zeros((n, dim))
+    jtj, grad = np.zeros((dim, dim)), np.zeros(dim)
+    _check(_calib_lib().ope_calib_jacobian(q.ctypes.data, n, model, p.ctypes.data, res.ctypes.data,
+                                           jac.ctypes.data, jtj.ctypes.data, grad.ctypes.data))
+    return res, jac, jtj, grad

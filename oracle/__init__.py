@@ -103,6 +103,11 @@ def lib():
     L.ora_bsm_greeks_many.argtypes = [_dp, C.c_int64, _dp]
     L.ora_implied_vol_many.restype = None
     L.ora_implied_vol_many.argtypes = [_dp, C.c_int64, _dp]
+    for nm in ("ora_calib_residuals", "ora_calib_jacobian", "ora_calib_jtj_grad"):
+        getattr(L, nm).restype = None
+    L.ora_calib_residuals.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp]
+    L.ora_calib_jacobian.argtypes = [_dp, C.c_int64, C.c_int, _dp, _dp]
+    L.ora_calib_jtj_grad.argtypes = [_dp, _dp, _dp, C.c_int64, C.c_int, _dp, _dp]
     L.ora_cf_greeks.restype = None
     L.ora_cf_greeks.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int] + [C.c_double] * 3 + [C.c_int, _dp]
     L.ora_cf_price_many.restype = None
@@ -235,6 +240,23 @@ def bsm_greeks_many(args, n_threads=1):
 def implied_vol_many(args, n_threads=1):
     """Rows (spot, strike, expiry, rate, div_yield, opt_type, price) -> vols (NaN: none)."""
     return _threaded_rows(lib().ora_implied_vol_many, args, 7, 1, n_threads)
+
+
+def calib_eval(quotes, model, params):
+    """calibration.rs residuals (573-587), FD Jacobian (613-652) and J'J / J'r
+    (655-667).  quotes: n x 8 rows (spot, strike, expiry, rate, div_yield,
+    opt_type, iv_market, weight); model 0 Heston, 1 Bates (every column
+    differenced).  Returns (residuals, jacobian n x dim, jtj, grad)."""
+    q = np.ascontiguousarray(quotes, dtype=np.float64).reshape(-1, 8)
+    n, dim = q.shape[0], 8 if model else 5
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    res, jac = np.zeros(n), np.zeros((n, dim))
+    jtj, grad = np.zeros((dim, dim)), np.zeros(dim)
+    L = lib()
+    L.ora_calib_residuals(q, n, model, p, res)
+    L.ora_calib_jacobian(q, n, model, p, jac)
+    L.ora_calib_jtj_grad(jac, res, np.ascontiguousarray(q[:, 7]), n, dim, jtj, grad)
+    return res, jac, jtj, grad
 
 
 def cf_price_many(args, n_threads=1):

@@ -204,3 +204,36 @@ def test_reference_greeks_signs(ope):
     b = ope.bates_price_and_greeks(100.0, 100.0, 1.0, 0.05, 0.0, ope.BatesParams(p, 0.0, 0.0, 1e-8))
     h = ope.heston_price_and_greeks(100.0, 100.0, 1.0, 0.05, 0.0, p)
     assert abs(b.delta - h.delta) < 0.01 and abs(b.vega - h.vega) < 1.0
+
+
+def _quotes(ope, ora, hp, jumps=None):
+    rows = []
+    for t in (0.25, 0.75, 1.5):
+        for k in (85.0, 95.0, 100.0, 105.0, 120.0):
+            ot = 0 if k >= 100.0 else 1
+            h = ora.heston(*hp)
+            px = (ora.bates_price(100.0, k, t, 0.03, 0.0, h, *jumps, ot) if jumps
+                  else ora.heston_price(100.0, k, t, 0.03, 0.0, h, ot))
+            rows.append(((100.0, k, t, 0.03, 0.0, 0.0, ot, 0), ora.implied_vol(100.0, k, t, 0.03, 0.0, ot, px),
+                         1.0 + 0.1 * k / 100.0))
+    return np.array(rows, dtype=ope.CALIB_DTYPE)
+
+
+@pytest.mark.parametrize("model", ["heston", "bates"])
+def test_calibration_evaluations_match_the_oracle(ope, ora, model):
+    # calibration.rs:573-667 on the GPU against the oracle's restatement, at a
+    # parameter vector away from the one that generated the quotes
+    gen, jg = [0.04, 1.5, 0.05, 0.5, -0.7], [0.5, -0.1, 0.15]
+    p0, j0 = [0.05, 2.0, 0.04, 0.4, -0.6], [0.4, -0.08, 0.12]
+    q = _quotes(ope, ora, gen, jg if model == "bates" else None)
+    hp = ope.HestonParams(*p0)
+    params = ope.BatesParams(hp, *j0) if model == "bates" else hp
+    res, jac, jtj, grad = ope.calib_jacobian(q, params)
+    rows = np.column_stack([q["contract"][f] for f in ("spot", "strike", "expiry", "rate", "div_yield")]
+                           + [q["contract"]["opt_type"], q["iv_market"], q["weight"]])
+    want = ora.calib_eval(rows, 1 if model == "bates" else 0, p0 + (j0 if model == "bates" else []))
+    for got, ref, name in zip((res, jac, jtj, grad), want, ("residuals", "jacobian", "jtj", "grad")):
+        err = np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref)))
+        print(f"calibration {model} {name}: max rel |gpu - oracle| = {err:.2e}", flush=True)
+        assert err < 1e-7, (name, got, ref)
+    assert np.array_equal(ope.calib_residuals(q, params), res)

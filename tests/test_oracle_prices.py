@@ -279,3 +279,55 @@ def test_implied_vol_rejects_and_wings(ora):
         px = ora.bsm_price(100.0, k, 1.0, 0.03, 0.0, vol, 0)
         got = ora.implied_vol(100.0, k, 1.0, 0.03, 0.0, 0, px)
         assert got is not None and abs(got - vol) < 1e-6, (k, vol, got)
+
+
+def _calib_quotes(ora, hp, jumps=None):
+    rows = []
+    for t in (0.25, 0.75, 1.5):
+        for k in (85.0, 95.0, 100.0, 105.0, 120.0):
+            ot = 0 if k >= 100.0 else 1
+            px = (ora.bates_price(100.0, k, t, 0.03, 0.0, ora.heston(*hp), *jumps, ot) if jumps
+                  else ora.heston_price(100.0, k, t, 0.03, 0.0, ora.heston(*hp), ot))
+            rows.append([100.0, k, t, 0.03, 0.0, ot, ora.implied_vol(100.0, k, t, 0.03, 0.0, ot, px), 1.0 + 0.1 * k / 100])
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_calibration_residuals_vanish_at_the_generating_parameters(ora, model):
+    hp, jp = [0.05, 2.0, 0.04, 0.4, -0.6], [0.4, -0.08, 0.12]
+    q = _calib_quotes(ora, hp, jp if model else None)
+    res, jac, jtj, grad = ora.calib_eval(q, model, hp + (jp if model else []))
+    assert np.abs(res).max() < 1e-12
+    assert np.abs(grad).max() < 1e-9
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_calibration_jacobian_is_the_implicit_function_derivative(ora, model):
+    # calibration.rs:598-611: d iv / d p = (d price / d p) / vega_BSM(iv); the
+    # price derivative here is a central difference of the CF price with a
+    # step 100x smaller than the reference's bumps
+    p0 = [0.05, 2.0, 0.04, 0.4, -0.6] + ([0.4, -0.08, 0.12] if model else [])
+    q = _calib_quotes(ora, [0.04, 1.5, 0.05, 0.5, -0.7], [0.5, -0.1, 0.15] if model else None)
+    res, jac, jtj, grad = ora.calib_eval(q, model, p0)
+    bumps = [1e-4, 1e-3, 1e-4, 1e-3, 1e-4, 1e-4, 1e-4, 1e-4]
+
+    def price(row, p):
+        h = ora.heston(*p[:5])
+        args = (row[0], row[1], row[2], row[3], row[4])
+        return (ora.bates_price(*args, h, p[5], p[6], p[7], int(row[5])) if model
+                else ora.heston_price(*args, h, int(row[5])))
+
+    for i, row in enumerate(q):
+        iv = res[i] + row[6]
+        vega = ora.bsm_greeks(row[0], row[1], row[2], row[3], row[4], iv, int(row[5]))["vega"]
+        for c in range(len(p0)):
+            h = bumps[c] / 100.0
+            pu, pd = list(p0), list(p0)
+            pu[c] += h
+            pd[c] -= h
+            want = (price(row, pu) - price(row, pd)) / (2 * h) / vega
+            assert abs(jac[i, c] - want) < 2e-3 * max(1.0, abs(want)), (i, c, jac[i, c], want)
+    # J'J and J'r with weights w: W^2-weighted normal equations
+    w2 = q[:, 7] ** 2
+    assert np.allclose(jtj, (jac * w2[:, None]).T @ jac, rtol=1e-12, atol=1e-14)
+    assert np.allclose(grad, (jac * w2[:, None]).T @ res, rtol=1e-12, atol=1e-14)
