@@ -327,6 +327,66 @@ def bsm_iv(ope, wm, steps=5, with_cpu=True, n_bsm=1 << 22, n_iv=1 << 20):
     return out
 
 
+def calibration_eval(ope, steps=5, with_cpu=True):
+    """One Levenberg-Marquardt iteration's evaluations (residuals, Jacobian,
+    J'J, J'r) for a 41-strike x 8-expiry quote surface, Heston and Bates, on
+    the GPU, with the oracle's restatement of calibration.rs beside it."""
+    import numpy as np
+
+    gen = ope.HestonParams(0.04, 1.5, 0.04, 0.5, -0.7)
+    gen_b = ope.BatesParams(gen, *JUMPS)
+    out = {}
+    for name, g, p in (("heston", gen, ope.HestonParams(0.05, 2.0, 0.045, 0.4, -0.6)),
+                       ("bates", gen_b, ope.BatesParams(ope.HestonParams(0.05, 2.0, 0.045, 0.4, -0.6), 0.4, -0.08,
+                                                        0.12))):
+        chain = np.zeros(len(EXPIRIES) * len(STRIKES), dtype=ope.CONTRACT_DTYPE)
+        chain["spot"] = 100.0
+        chain["strike"] = np.tile(STRIKES, len(EXPIRIES))
+        chain["expiry"] = np.repeat(EXPIRIES, len(STRIKES))
+        chain["rate"] = 0.03
+        opts = np.zeros(len(chain), dtype=ope.CF_DTYPE)
+        for f in ("spot", "strike", "expiry", "rate", "div_yield", "opt_type"):
+            opts[f] = chain[f]
+        hp = g.heston if name == "bates" else g
+        for f in ("v0", "kappa", "theta", "sigma", "rho"):
+            opts[f] = getattr(hp, f)
+        if name == "bates":
+            opts["lambda_"], opts["mu_j"], opts["sigma_j"], opts["model"] = JUMPS[0], JUMPS[1], JUMPS[2], 1
+        px = ope.cf_price_options(opts)
+        iv, found, _ = ope.implied_vol_chain(chain, px)
+        q = np.zeros(len(chain), dtype=ope.CALIB_DTYPE)
+        q["contract"], q["iv_market"], q["weight"] = chain, np.where(found == 1, iv, 0.2), 1.0
+        ope.calib_jacobian(q, p)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res, jac, jtj, grad = ope.calib_jacobian(q, p)
+        wall = (time.perf_counter() - t0) / steps
+        dim = jac.shape[1]
+        entry = {"quotes": len(q), "parameters": dim, "prices_per_eval": len(q) * (1 + 2 * dim),
+                 "ms_per_eval": wall * 1e3}
+        if with_cpu:
+            import oracle
+
+            rows = np.column_stack([q["contract"][f] for f in ("spot", "strike", "expiry", "rate", "div_yield")]
+                                   + [q["contract"]["opt_type"], q["iv_market"], q["weight"]])
+            pv = [p.heston.v0, p.heston.kappa, p.heston.theta, p.heston.sigma, p.heston.rho, p.lambda_, p.mu_j,
+                  p.sigma_j] if name == "bates" else [p.v0, p.kappa, p.theta, p.sigma, p.rho]
+            t0 = time.perf_counter()
+            ref = oracle.calib_eval(rows, 1 if name == "bates" else 0, pv)
+            cpu = time.perf_counter() - t0
+            entry["max_rel_diff_vs_oracle"] = float(np.max(np.abs(jac - ref[1]) / np.maximum(1.0, np.abs(ref[1]))))
+            entry["cpu_baseline"] = {"ms_per_eval": cpu * 1e3, "cores": 1, "kind": "port",
+                                     "sample": "the same evaluation, C restatement of calibration.rs, one thread "
+                                               "(the reference's evaluation loop is sequential)"}
+        out[name] = entry
+        print(f"# calibration {name}: {wall * 1e3:.2f} ms per evaluation ({entry['prices_per_eval']} prices)",
+              file=sys.stderr, flush=True)
+    out["workload"] = ("residuals, FD Jacobian, J'J and J'r of 328 quotes (41 strikes 60..140 x 8 expiries "
+                       "0.1..2, implied vols of the Heston / Bates prices at v0=theta=0.04, kappa=1.5, sigma=0.5, "
+                       "rho=-0.7 [, lambda=0.5, mu_j=-0.1, sigma_j=0.15]) at a parameter vector away from it")
+    return out
+
+
 def serving(ope, seconds):
     """The serving configuration (Poisson arrivals at 2000 req/s) in both
     precisions, plus the same requests all at once (saturated throughput)."""
@@ -438,7 +498,8 @@ def main():
         return
     if args.workload == "analytic_cf":
         print(json.dumps({"analytic_cf": analytic_cf(ope, wm, steps=args.steps, with_cpu=not args.no_cpu),
-                          "bsm_iv": bsm_iv(ope, wm, steps=args.steps, with_cpu=not args.no_cpu)}))
+                          "bsm_iv": bsm_iv(ope, wm, steps=args.steps, with_cpu=not args.no_cpu),
+                          "calibration_eval": calibration_eval(ope, steps=args.steps, with_cpu=not args.no_cpu)}))
         return
     desc, conf, run, path_steps, flops = make_workload(ope, wm, args.workload)
 
@@ -489,6 +550,7 @@ def main():
             line[extra] = secondary(ope, wm, extra)
         line["analytic_cf"] = analytic_cf(ope, wm, with_cpu=not args.no_cpu)
         line["bsm_iv"] = bsm_iv(ope, wm, with_cpu=not args.no_cpu)
+        line["calibration_eval"] = calibration_eval(ope, with_cpu=not args.no_cpu)
     if not args.no_serve:
         line["serving"] = serving(ope, args.serve_seconds)
     if not args.no_cpu:
