@@ -120,25 +120,33 @@ __device__ __forceinline__ PairStream make_stream(const McDesc& d, uint64_t pair
     return s;
 }
 
-// Jump of step s given its Poisson word: jump_sum = mu_j n + sigma_j sqrt(n) z
-// (mc.rs:139-140).  FP64 mode returns it in ln units; FP32 mode in log2 units.
+// Jump of step s given the 16 high bits h of its Poisson uniform word:
+// jump_sum = mu_j n + sigma_j sqrt(n) z (mc.rs:139-140).  h < thr0 >> 16
+// means count 0 whatever the low bits; otherwise (probability ~ lambda dt)
+// the jump call of step s supplies the low 16 bits (word 2), completing the
+// 32-bit word W of the exact inverse-CDF count, and the jump normal (the
+// Box-Muller cosine of words 0, 1) when the count is non-zero.  FP64 mode
+// returns the jump in ln units, FP32 mode in log2 units.
 template <bool FP32>
-__device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& st, uint32_t s, uint32_t wp,
+__device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& st, uint32_t s, uint32_t h,
                                              uint32_t& n) {
     using R = typename std::conditional<FP32, float, double>::type;
     R j = 0;
     n = 0;
-    if (wp >= d.pois_thr0) {  // rare: P = 1 - e^{-lambda dt}
-        n = poisson_tail(wp, d.pois_mu, d.pois_e0);
+    if (h >= d.pois_thr0_hi) {  // rare: P ~ 1 - e^{-lambda dt} + 2^-16
         const uint4 w = st.call(s, kTagJump);
-        if constexpr (FP32) {
-            float z, unused;
-            box_muller_f32(w.x, w.y, z, unused);
-            j = fmaf(d.f_sigma_j2 * hw_sqrt((float)n), z, d.f_mu_j2 * (float)n);
-        } else {
-            double z, unused;
-            box_muller_f64(w.x, w.y, z, unused);
-            j = d.mu_j * (double)n + d.sigma_j * sqrt((double)n) * z;
+        const uint32_t W = (h << 16) | (w.z & 0xFFFFu);
+        if (W >= d.pois_thr0) {
+            n = poisson_tail(W, d.pois_mu, d.pois_e0);
+            if constexpr (FP32) {
+                float z, unused;
+                box_muller_f32(w.x, w.y, z, unused);
+                j = fmaf(d.f_sigma_j2 * hw_sqrt((float)n), z, d.f_mu_j2 * (float)n);
+            } else {
+                double z, unused;
+                box_muller_f64(w.x, w.y, z, unused);
+                j = d.mu_j * (double)n + d.sigma_j * sqrt((double)n) * z;
+            }
         }
     }
     return j;
@@ -146,8 +154,9 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
 
 // Drives the Philox-mode draws of one pair through all steps, calling
 // step(za, zb, jump, n, s) once per step s.  Heston: one Philox call per two
-// steps.  Bates: the same normals plus one Poisson call per four steps and,
-// only for a step whose count is non-zero, one jump call.
+// steps.  Bates: the same normals plus one Poisson call per eight steps (16
+// bits per step) and, only for a step whose count may be non-zero, one jump
+// call.
 //
 // DEFER (FP32 European payoffs, whose log-price needs only the sum of the
 // jumps): step() gets jump 0, and a step with a non-zero count only records
@@ -186,7 +195,7 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
         uint32_t n = 0;
         R j = 0;
         if constexpr (DEFER) {
-            if (wp >= d.pois_thr0) {
+            if (wp >= d.pois_thr0_hi) {
                 if (es0 == ~0u) {
                     es0 = s;
                     ew0 = wp;
@@ -206,23 +215,32 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     const uint32_t N = d.n_steps;
     uint32_t i = 0;
     if constexpr (JUMPS) {
-        for (; i + 4 <= N; i += 4) {
-            const uint4 p = st.call(i >> 2, kTagPoisson);
-            uint4 w = st.call(i >> 1, kTagNormal);
-            one(w.x, w.y, p.x, i);
-            one(w.z, w.w, p.y, i + 1);
-            w = st.call((i >> 1) + 1, kTagNormal);
-            one(w.x, w.y, p.z, i + 2);
-            one(w.z, w.w, p.w, i + 3);
+        // Poisson halves of step s: word (s >> 1) & 3 of call s >> 3, high
+        // half for even s, low half for odd s
+        auto duo = [&](uint32_t pw, uint32_t s) {  // steps s (even) and s + 1
+            const uint4 w = st.call(s >> 1, kTagNormal);
+            one(w.x, w.y, pw >> 16, s);
+            one(w.z, w.w, pw & 0xFFFFu, s + 1);
+        };
+        for (; i + 8 <= N; i += 8) {
+            const uint4 p = st.call(i >> 3, kTagPoisson);
+            duo(p.x, i);
+            duo(p.y, i + 2);
+            duo(p.z, i + 4);
+            duo(p.w, i + 6);
         }
-        if (i < N) {
-            const uint4 p = st.call(i >> 2, kTagPoisson);
-            uint4 w = st.call(i >> 1, kTagNormal);
-            one(w.x, w.y, p.x, i);
-            if (i + 1 < N) one(w.z, w.w, p.y, i + 1);
-            if (i + 2 < N) {
-                w = st.call((i >> 1) + 1, kTagNormal);
-                one(w.x, w.y, p.z, i + 2);
+        if (i < N) {  // 1..7 steps left
+            const uint4 p = st.call(i >> 3, kTagPoisson);
+            const uint32_t pw[4] = {p.x, p.y, p.z, p.w};
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t s = i + 2u * k;
+                if (s + 1 < N) {
+                    duo(pw[k], s);
+                } else if (s < N) {
+                    const uint4 w = st.call(s >> 1, kTagNormal);
+                    one(w.x, w.y, pw[k] >> 16, s);
+                }
             }
         }
     } else {

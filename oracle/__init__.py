@@ -372,10 +372,13 @@ def philox_draws(seed, pair_ids, n_steps, dt=None, lam=0.0, mu_j=0.0, sigma_j=0.
     Counter = (call, pair lo, pair hi, tag), key = (seed lo, seed hi).
     Normals of step s: call s >> 1 with tag 0, words (0,1) for even s and
     (2,3) for odd s, mapped by Box-Muller.  Bates (lam > 0) adds the Poisson
-    count of step s, by inverse CDF of mean lam*dt, from word s % 4 of call
-    s >> 2 with tag 1, and, when that count is non-zero, the jump normal from
-    the Box-Muller cosine output of words (0,1) of call s with tag 2.
-    Returns za, zb, jump_sum, n as [n_steps, n_pairs].
+    count of step s, by inverse CDF of mean lam*dt of the 32-bit word
+    W = (h << 16) | l: h is the high half (even s) or low half (odd s) of
+    word (s >> 1) & 3 of call s >> 3 with tag 1; when h >= T >> 16 (T the
+    count-0 threshold) l is the low 16 bits of word 2 of call s with tag 2,
+    otherwise the count is 0 whatever l is.  A non-zero count takes its jump
+    normal from the Box-Muller cosine output of words (0,1) of that tag-2
+    call.  Returns za, zb, jump_sum, n as [n_steps, n_pairs].
     """
     pair_ids = np.asarray(pair_ids, dtype=np.uint64)
     npairs = pair_ids.size
@@ -398,11 +401,15 @@ def philox_draws(seed, pair_ids, n_steps, dt=None, lam=0.0, mu_j=0.0, sigma_j=0.
         za[s], zb[s] = _box_muller(wa, wb)
         if bates:
             mu = lam * dt
-            p = call(s >> 2, TAG_POISSON)[s & 3]
-            n = poisson_from_word(p, mu)
-            nn[s] = n
-            if np.any(n > 0):
+            word = call(s >> 3, TAG_POISSON)[(s >> 1) & 3]
+            h = (word >> np.uint64(16)) if (s & 1) == 0 else (word & np.uint64(0xFFFF))
+            thr_hi = min(poisson_threshold0(mu), M32) >> 16
+            maybe = h >= np.uint64(thr_hi)
+            if np.any(maybe):
                 wj = call(s, TAG_JUMP)
+                full = (h << np.uint64(16)) | (wj[2] & np.uint64(0xFFFF))
+                n = np.where(maybe, poisson_from_word(full, mu), 0)
                 zj, _ = _box_muller(wj[0], wj[1])
                 js[s] = np.where(n > 0, mu_j * n + sigma_j * np.sqrt(n) * zj, 0.0)
+                nn[s] = n
     return za, zb, js, nn
