@@ -415,15 +415,43 @@ def time_mode(run, precision, steps, warmup):
     return prices, ses, wall / steps * 1e3, sum(kms) / len(kms)
 
 
-def cpu_baseline(workload):
+def cpu_uneven_sample(ope, cores, budget=6e8):
+    """The uneven batch's requests in order, each priced by the C restatement
+    of mc.rs on `cores` threads, until `budget` path-steps have been done."""
+    import oracle
+
+    reqs = draw_uneven_requests(ope, 256, 4)
+    done = n_req = 0
+    t0 = time.perf_counter()
+    for q in reqs:
+        hp = oracle.heston(q.params.v0, q.params.kappa, q.params.theta, q.params.sigma, q.params.rho)
+        pay = oracle.payoff(q.payoff.kind, q.payoff.strike, int(q.payoff.opt_type), q.payoff.barrier,
+                            q.payoff.rebate)
+        jp = oracle.jumps(q.lambda_, q.mu_j, q.sigma_j) if q.lambda_ > 0 else None
+        oracle.mc_price(q.spot, q.expiry, q.rate, q.div_yield, hp, pay, q.cfg.n_paths, q.cfg.n_steps, q.cfg.seed,
+                        True, oracle.EULER, jp, n_threads=cores)
+        done += q.cfg.n_paths * q.cfg.n_steps
+        n_req += 1
+        if done >= budget:
+            break
+    wall = time.perf_counter() - t0
+    return {"value": done / wall, "unit": "path-steps/s", "cores": cores, "kind": "port",
+            "sample": f"the first {n_req} of the 256 requests ({done:.3g} path-steps), one mc_price call each; "
+                      f"{cores} threads, chunk seeding as mc.rs:159-172; {wall:.2f} s wall"}
+
+
+def cpu_baseline(workload, ope=None):
     """C restatement of mc.rs (oracle/), FP64, on this host's cores, bounded sample."""
     import oracle
 
     cores = max(1, min(16, os.cpu_count() or 1))
+    if workload == "uneven_batch":
+        return cpu_uneven_sample(ope, cores)
     hp = oracle.heston(**HP)
     if workload == "bates_surface":
-        n_paths, n_steps, scheme, pay, jp = 200_000, 252, oracle.EULER, oracle.payoff(), oracle.jumps(*JUMPS)
-        sample = f"Bates Euler European K=100 T=1, {n_paths} paths x {n_steps} steps (one of the 328 surface calls)"
+        n_paths, n_steps, scheme, pay, jp = 1_000_000, 252, oracle.EULER, oracle.payoff(), oracle.jumps(*JUMPS)
+        sample = (f"Bates Euler European K=100 T=1, {n_paths} paths x {n_steps} steps (a quarter of one of the "
+                  "328 per-strike calls the reference makes for the surface)")
     elif workload == "heston_euler":
         n_paths, n_steps, scheme, pay, jp = 100_000, 252, oracle.EULER, oracle.payoff(), None
         sample = f"Heston Euler European, {n_paths} paths x {n_steps} steps (the full workload)"
@@ -546,15 +574,24 @@ def main():
         "serving": None,  # filled below
     }
     if args.workload == "qe_asian_uao" and not args.no_extra:
-        for extra in ("bates_surface", "uneven_batch"):
+        for extra in ("bates_surface", "uneven_batch", "heston_euler"):
             line[extra] = secondary(ope, wm, extra)
+            if not args.no_cpu:
+                cb = cpu_baseline(extra, ope)
+                # the whole workload on the host at the sampled rate; the
+                # reference prices the surface as 328 separate (expiry, strike)
+                # calls, each simulating its own 4M paths: 41x the path-steps
+                # of the shared-path surface
+                resim = len(STRIKES) if extra == "bates_surface" else 1
+                cb["extrapolated_workload_s"] = line[extra]["path_steps"] * resim / cb["value"]
+                line[extra]["cpu_baseline"] = cb
         line["analytic_cf"] = analytic_cf(ope, wm, with_cpu=not args.no_cpu)
         line["bsm_iv"] = bsm_iv(ope, wm, with_cpu=not args.no_cpu)
         line["calibration_eval"] = calibration_eval(ope, with_cpu=not args.no_cpu)
     if not args.no_serve:
         line["serving"] = serving(ope, args.serve_seconds)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.workload)
+        line["cpu_baseline"] = cpu_baseline(args.workload, ope)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))

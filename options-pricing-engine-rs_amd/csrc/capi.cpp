@@ -195,8 +195,9 @@ int make_desc(const Request& r, McDesc& d) {
         d.qe_q0 = (float)(kLog2eD * d.qe_a0);
         d.qe_q1 = (float)(kLog2eD * d.k1);
         d.qe_q2 = (float)(kLog2eD * d.k2);
-        d.qe_q3 = (float)(kLog2eD * kLog2eD * d.k3);
-        d.qe_q4 = (float)(kLog2eD * kLog2eD * d.k4);
+        // k3 == k4 (gamma1 == gamma2), so the FP32 path's variance term is
+        // sqrt(max(k3, 0)) sqrt(v + v'): the clamp at 0 only ever acts for k3 < 0
+        d.qe_sq3 = (float)(kLog2eD * std::sqrt(std::fmax(d.k3, 0.0)));
     }
     d.f_v0 = (float)hp.v0;
     d.eu_a0 = (float)(kLog2eD * drift * dt);

@@ -58,8 +58,9 @@ struct alignas(16) McDesc {
     float f_rho, f_rhoc;           // z2 = rho za + sqrt(1-rho^2) zb
     float qe_m1, qe_m0;            // m = m1 v+ + m0
     float qe_s1, qe_s0;            // s2 = s1 v+ + s0
-    float qe_q0, qe_q1, qe_q2;     // dy = q0 + q1 v+ + q2 v' + sqrt(max(q3 v+ + q4 v', 0)) zb
-    float qe_q3, qe_q4;
+    float qe_q0, qe_q1, qe_q2;     // dy = q0 + q1 v+ + q2 v' + sq3 sqrt(v+ + v') zb
+    float qe_sq3;                  // sqrt(max(q3, 0)), q3 = q4 the log2-scaled K3 = K4
+    float pad_f_;
     float f_mu_j2, f_sigma_j2;     // log2e * mu_j, log2e * sigma_j
 };
 

@@ -353,65 +353,71 @@ __device__ __forceinline__ float ncdf_neg_abs_f32(float z) {
     return t * hw_exp2(fmaf(-xs, xs, p));
 }
 
-// QE variance step in FP32 (Andersen 2008, psi_c = 1.5), in terms of the
-// conditional mean m, mm2 = 2 m^2 and the conditional variance s2, with no
-// division by psi = s2/m^2:
-// * quadratic branch: 2/psi = mm2/s2, capped at 2e12 (the psi floor 1e-12),
-//   b^2 = 2/psi - 1 + sqrt(2/psi (2/psi - 1)), a = m/(1 + b^2),
-//   v' = a (b + z)^2 (meaningless when psi > 1.5, and then replaced);
-// * exponential branch, given phic = 1 - ncdf(z): 1 - p = mm2/(s2 + m^2),
-//   1/beta = m/(1 - p); v' = 0 when u <= p (phic >= 1 - p), else
-//   ln((1 - p)/phic)/beta, which avoids the cancellation of 1 - u.
-__device__ __forceinline__ float qe_quad_f32(float m, float mm2, float s2, float z) {
-    const float t2 = fminf(mm2 * hw_rcp(s2), 2e12f);
-    const float a = t2 - 1.0f;
-    const float b2 = a + hw_sqrt(t2 * a);
-    const float bz = hw_sqrt(b2) + z;
-    return (m * hw_rcp(1.0f + b2)) * (bz * bz);
-}
+// QE variance step in FP32 (Andersen 2008, psi_c = 1.5) for one sibling, in
+// terms of the conditional mean m, mm2 = 2 m^2 and the conditional variance
+// s2, with no division by psi = s2/m^2; psi > 1.5 <=> s2 > 0.75 mm2.  The two
+// branches share their reciprocals and the final product:
+// * quadratic: t = 2/psi = mm2/s2 (capped at 2e12, the psi floor 1e-12),
+//   b^2 = t - 1 + sqrt(t (t - 1)), v' = m/(1 + b^2) (b + z)^2;
+// * exponential: t = 1 - p = mm2/(s2 + m^2), 1/beta = m/(1 - p), and with
+//   phic = 1 - ncdf(z): v' = 0 when u <= p (phic >= t), else
+//   ln(t/phic)/beta = m/t max(ln(t/phic), 0), which avoids the cancellation
+//   of 1 - u.
+// qe_sib_f32 returns m r and (b + z)^2 with r = 1/(1 + b^2) or 1/t; the
+// exponential lanes' (b + z)^2 is meaningless and is replaced by the caller.
+struct QeSib {
+    float mr, bz2, t;
+    bool ex;
+};
 
-__device__ __forceinline__ float qe_expo_f32(float m, float mm2, float s2, float phic) {
-    const float q = mm2 * hw_rcp(fmaf(0.5f, mm2, s2));
-    const float v = (hw_log2(q * hw_rcp(phic)) * 0.69314718055994531f) * (m * hw_rcp(q));
-    return phic >= q ? 0.0f : v;
+__device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z) {
+    QeSib o;
+    const float m = fmaf(d.qe_m1, v, d.qe_m0), s2 = fmaf(d.qe_s1, v, d.qe_s0);
+    const float mm2 = 2.0f * (m * m);
+    o.ex = s2 > 0.75f * mm2;
+    o.t = fminf(mm2 * hw_rcp(o.ex ? fmaf(0.5f, mm2, s2) : s2), 2e12f);
+    const float a = o.t - 1.0f;
+    const float b2 = a + hw_sqrt(o.t * a);
+    const float bz = hw_sqrt(b2) + z;
+    o.bz2 = bz * bz;
+    o.mr = m * hw_rcp(o.ex ? o.t : 1.0f + b2);
+    return o;
 }
 
 // One QE step of both siblings of a pair: sibling a draws (za, zb), sibling b
 // (-za, -zb).  |za| is common to both, so the exponential branch's erfc is
 // evaluated once per pair, and only when a lane of the wave needs it.  The
 // variances are >= 0 on entry (QE's v' is never negative, and v0 is clamped
-// by the caller).  Returns the log2-price increments.
+// by the caller).  The price variance term K3 v + K4 v' has K3 = K4
+// (gamma1 = gamma2 = 1/2), so sqrt(max(K3 v + K4 v', 0)) =
+// sqrt(max(K3, 0)) sqrt(v + v'), and the constant folds into zb.  Returns the
+// log2-price increments.
 template <bool JUMPS>
 __device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& vb, bool anti, float za, float zb,
                                             float jl, float& dya, float& dyb) {
-    const float ma = fmaf(d.qe_m1, va, d.qe_m0), sa = fmaf(d.qe_s1, va, d.qe_s0);
-    const float ma2 = 2.0f * (ma * ma);
-    float na = qe_quad_f32(ma, ma2, sa, za);
-    const bool xa = sa > 0.75f * ma2;  // psi > 1.5
-    float mb = 0.0f, sb = 0.0f, mb2 = 0.0f, nb = 0.0f;
-    bool xb = false;
-    if (anti) {
-        mb = fmaf(d.qe_m1, vb, d.qe_m0);
-        sb = fmaf(d.qe_s1, vb, d.qe_s0);
-        mb2 = 2.0f * (mb * mb);
-        nb = qe_quad_f32(mb, mb2, sb, -za);
-        xb = sb > 0.75f * mb2;
+    const QeSib a = qe_sib_f32(d, va, za);
+    QeSib b = {0.0f, 0.0f, 0.0f, false};
+    if (anti) b = qe_sib_f32(d, vb, -za);
+    float la = 0.0f, lb = 0.0f;
+    if (a.ex || b.ex) {
+        const float e = ncdf_neg_abs_f32(za), ome = 1.0f - e;
+        const bool pos = za >= 0.0f;
+        // ln(t / (1 - ncdf(z))) for z = za (sibling a) and z = -za (sibling b)
+        la = fmaxf(hw_log2(a.t * hw_rcp(pos ? e : ome)), 0.0f) * 0.69314718055994531f;
+        lb = fmaxf(hw_log2(b.t * hw_rcp(pos ? ome : e)), 0.0f) * 0.69314718055994531f;
     }
-    if (xa || xb) {
-        const float e = ncdf_neg_abs_f32(za);
-        if (xa) na = qe_expo_f32(ma, ma2, sa, za >= 0.0f ? e : 1.0f - e);  // 1 - ncdf(za)
-        if (xb) nb = qe_expo_f32(mb, mb2, sb, za <= 0.0f ? e : 1.0f - e);  // 1 - ncdf(-za)
-    }
+    const float na = a.mr * (a.ex ? la : a.bz2);
+    const float zs = d.qe_sq3 * zb;
     auto incr = [&](float vp, float vn, float z) {
-        const float var = fmaxf(fmaf(d.qe_q3, vp, d.qe_q4 * vn), 0.0f);
-        float dy = fmaf(hw_sqrt(var), z, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, d.qe_q0)));
+        float dy = fmaf(hw_sqrt(vp + vn), z, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, d.qe_q0)));
         if (JUMPS) dy += jl;
         return dy;
     };
-    dya = incr(va, na, zb);
+    dya = incr(va, na, zs);
     va = na;
     if (anti) {
-        dyb = incr(vb, nb, -zb);
+        const float nb = b.mr * (b.ex ? lb : b.bz2);
+        dyb = incr(vb, nb, -zs);
         vb = nb;
     } else {
         dyb = 0.0f;
