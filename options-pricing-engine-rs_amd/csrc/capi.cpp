@@ -65,11 +65,19 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Launch groups of one call beyond the first run on these streams, so that
+// a group's last tiles overlap the next group's first ones instead of
+// leaving CUs idle between two kernels (the stream, plus these, stays
+// within the 4 hardware queues a process gets by default).
+constexpr int kAuxStreams = 3;
+
 struct Context {
     std::mutex mu;
     bool tried = false;
     int init_status = OPE_OK;
     hipStream_t stream = nullptr;
+    hipStream_t aux[kAuxStreams] = {};
+    hipEvent_t join[kAuxStreams] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
     DevBuf descs, tiles, strikes, slots, jobs, out;
     DevBuf rz_a, rz_b, rz_j, path_pay, path_state;
@@ -92,6 +100,13 @@ int ensure_init(Context& c) {
         (e = hipEventCreate(&c.evk)) != hipSuccess) {
         c.init_status = fail(OPE_ERR_NO_DEVICE, "HIP initialisation failed: %s", hipGetErrorString(e));
         return c.init_status;
+    }
+    for (int i = 0; i < kAuxStreams; ++i) {
+        if ((e = hipStreamCreateWithFlags(&c.aux[i], hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&c.join[i], hipEventDisableTiming)) != hipSuccess) {
+            c.init_status = fail(OPE_ERR_NO_DEVICE, "HIP initialisation failed: %s", hipGetErrorString(e));
+            return c.init_status;
+        }
     }
     c.init_status = OPE_OK;
     return OPE_OK;
@@ -341,7 +356,14 @@ int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
     HIP_TRY(c.slots.ensure(p.n_slots * sizeof(double2)));
     HIP_TRY(c.out.ensure(p.n_out * sizeof(double2)));
     HIP_TRY(hipEventRecord(c.ev0, s));
-    for (const Plan::Group& g : p.groups) {
+    // group i runs on stream i mod (1 + kAuxStreams); the auxiliary streams
+    // start after the uploads (ev0) and the reduction waits for all of them.
+    // Every tile writes its own slots, so the overlap changes no result.
+    const int n_aux = std::min<int>(kAuxStreams, (int)p.groups.size() - 1);
+    for (int i = 0; i < n_aux; ++i) HIP_TRY(hipStreamWaitEvent(c.aux[i], c.ev0, 0));
+    for (size_t gi = 0; gi < p.groups.size(); ++gi) {
+        const Plan::Group& g = p.groups[gi];
+        const size_t lane = gi % (size_t)(1 + kAuxStreams);
         TileLaunch L;
         L.descs = c.descs.as<McDesc>();
         L.tiles = c.tiles.as<uint2>() + g.first;
@@ -351,7 +373,11 @@ int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
         L.fp32 = fp32;
         L.fixed = g.code < kNumCodes ? 1 : 0;
         L.code = g.code;
-        HIP_TRY(launch_tiles(L, s));
+        HIP_TRY(launch_tiles(L, lane == 0 ? s : c.aux[lane - 1]));
+    }
+    for (int i = 0; i < n_aux; ++i) {
+        HIP_TRY(hipEventRecord(c.join[i], c.aux[i]));
+        HIP_TRY(hipStreamWaitEvent(s, c.join[i], 0));
     }
     HIP_TRY(hipEventRecord(c.evk, s));
     HIP_TRY(launch_reduce(c.descs.as<McDesc>(), c.jobs.as<uint2>(), (uint32_t)p.jobs.size(), c.slots.as<double2>(),
