@@ -32,11 +32,25 @@ constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
 
 // Substreams of one pair (the counter's fourth word):
 //   kTagNormal   call s/2: words (x,y) -> normals of step s for even s, (z,w) for odd s
-//   kTagPoisson  call s/4: word s%4 -> Poisson count of step s (Bates)
-//   kTagJump     call s:   words (x,y) -> the jump-size normal of step s, drawn
-//                          only when its Poisson count is non-zero
+//   kTagPoisson  call s/8: word (s/2)%4, high half for even s, low half for
+//                odd s -> the 16 high bits of the Poisson uniform of step s (Bates)
+//   kTagJump     call s:   word z -> its 16 low bits, when the high bits
+//                          allow a non-zero count; words (x,y) -> the
+//                          jump-size normal of step s when the count is non-zero
 constexpr uint32_t kTagNormal = 0u, kTagPoisson = 1u, kTagJump = 2u;
 
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96, the parity of the three
+// inputs).  The instruction takes at most one scalar operand, so it is used
+// only where the other two are per-lane values.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// In the engine's counters (call, pair lo, pair hi, tag) the call index, the
+// pair's high word and the tag are uniform across a wave, so round 0 has
+// uniform multiplicands and round 1's x input is uniform: their xors stay
+// plain (the compiler runs the uniform ones on the scalar unit).  From round
+// 2 on every word is per-lane, and each output word is one bitop3.
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -44,7 +58,10 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        if (r < 2)
+            c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        else
+            c = make_uint4(xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0);
         k0 += kPhiloxW0;
         k1 += kPhiloxW1;
     }
