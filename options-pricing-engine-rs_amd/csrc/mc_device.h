@@ -78,6 +78,14 @@ __device__ __forceinline__ float hw_cos_rev(float x) { return __builtin_amdgcn_c
 __device__ __forceinline__ float clamp_pos(float x) {                                     // max(x, 0), one v_med3
     return __builtin_amdgcn_fmed3f(x, 0.0f, 3.402823466e38f);
 }
+// A uniform constant kept in a vector register for a whole loop.  A VOP3
+// instruction reads at most one scalar operand, so an FMA of two uniform
+// constants and a per-lane value would otherwise copy one of the constants
+// into a vector register at every step.
+__device__ __forceinline__ float in_vgpr(float x) {
+    asm("" : "+v"(x));
+    return x;
+}
 
 constexpr double kLog2e = 1.4426950408889634;
 constexpr double kLn2 = 0.6931471805599453;
@@ -196,6 +204,7 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
     uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u;
     float jd = 0.0f;
+    const float phi = MIX ? in_vgpr(d.eu_phi) : 0.0f;
     auto one = [&](uint32_t wa, uint32_t wb, uint32_t wp, uint32_t s) {
         R za, zb;
         if constexpr (MIX) {
@@ -203,7 +212,7 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
             const float wf = (float)wb;
             const float ra = hw_sqrt(d.eu_r2 * hw_log2(u1));  // a2 sqrt(-2 ln u1)
             za = ra * hw_cos_rev(wf * 0x1p-32f);
-            zb = (ra * d.eu_kb) * hw_cos_rev(fmaf(wf, 0x1p-32f, -d.eu_phi));
+            zb = (ra * d.eu_kb) * hw_cos_rev(fmaf(wf, 0x1p-32f, -phi));
         } else if constexpr (FP32) {
             box_muller_f32(wa, wb, za, zb);
         } else {
@@ -335,15 +344,17 @@ __device__ __forceinline__ void qe_f64(const McDesc& d, double& ln_s, double& v,
 //
 // Euler for one sibling in FP32: sa = a2*z1 and sb = b2*z2 are per-pair
 // products (the antithetic sibling passes them negated).  v >= 0 on entry:
-// every step ends with max(., 0), and a negative v0 takes the first step in
-// sim_pair_f32 instead.  So v+ = v and v + b0 + b1 v+ = c1 v + b0 with
-// c1 = 1 + b1.  Returns acc + a1 v + sqrt(v) sa: the caller passes a0 (+ the
-// jump) for the increment itself, or a running partial sum to which the
-// constant a0 is added once per path.
-__device__ __forceinline__ float euler_f32(const McDesc& d, float& v, float sa, float sb, float acc) {
+// every step ends with max(., 0), and a negative v0 enters as 0 (see
+// sim_pair_f32).  So v+ = v and v + b0 + b1 v+ = c1 v + b0 with c1 = 1 + b1
+// (c1 passed by the caller, held in a vector register).  Returns
+// acc + a1 v + sqrt(v) sa: the caller passes a0 (+ the jump) for the
+// increment itself, or a running partial sum to which the constant a0 is
+// added once per path.
+__device__ __forceinline__ float euler_f32(const McDesc& d, float& v, float sa, float sb, float acc, float c1,
+                                           float b0) {
     const float sv = hw_sqrt(v);
     const float r = fmaf(sv, sa, fmaf(d.eu_a1, v, acc));
-    v = fmaxf(fmaf(sv, sb, fmaf(d.eu_c1, v, d.eu_b0)), 0.0f);
+    v = fmaxf(fmaf(sv, sb, fmaf(c1, v, b0)), 0.0f);
     return r;
 }
 
@@ -547,8 +558,9 @@ template <int SCHEME, bool JUMPS, int PAYOFF, class Src>
 __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bool anti, PathState& ea,
                                              PathState& eb) {
     double ya = 0.0, yb = 0.0;
-    // QE only ever reads v+ = max(v, 0) and never produces a negative v
-    float va = SCHEME == kSchemeEuler ? d.f_v0 : clamp_pos(d.f_v0), vb = va;
+    // QE only ever reads v+ = max(v, 0) and never produces a negative v; the
+    // Euler step's handling of a negative v0 is below
+    float va = clamp_pos(d.f_v0), vb = va;
     float suma = 0.0f, sumb = 0.0f;
     double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
     const uint32_t N = d.n_steps;
@@ -570,7 +582,12 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
             y += (double)dy;
         }
     };
-    const bool v0_neg = SCHEME == kSchemeEuler && d.f_v0 < 0.0f;
+    // A negative v0 has v+ = 0 in the first Euler step (mc.rs:245-254): no
+    // diffusion and v' = max(v0 + b0, 0).  The general step computes exactly
+    // that from v = 0 with b0 + v0 in place of b0, so the first step uses b0s
+    // and every later one b0.
+    float b0s = d.f_v0 < 0.0f ? d.f_v0 + d.eu_b0 : d.eu_b0;
+    const float c1 = in_vgpr(d.eu_c1);
     constexpr bool kMixed = Src::kPhilox && SCHEME == kSchemeEuler;
     auto step = [&](float za, float zb, float jl, uint32_t, uint32_t s) {
         if (SCHEME == kSchemeEuler) {
@@ -579,15 +596,10 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
             // which costs nothing more than selecting it out
             const float sa = kMixed ? za : d.eu_a2 * za;
             const float sb = kMixed ? zb : d.eu_b2 * fmaf(d.f_rho, za, d.f_rhoc * zb);
-            const bool first_neg = v0_neg && s == 0;  // v+ = 0: no diffusion, v' = max(v0 + b0, 0)
             if (PAYOFF == kPayEuropean) {
                 if (JUMPS) jsum += jl;
-                if (first_neg) {
-                    va = vb = fmaxf(d.f_v0 + d.eu_b0, 0.0f);
-                } else {
-                    pa = euler_f32(d, va, sa, sb, pa);
-                    pb = euler_f32(d, vb, -sa, -sb, pb);
-                }
+                pa = euler_f32(d, va, sa, sb, pa, c1, b0s);
+                pb = euler_f32(d, vb, -sa, -sb, pb, c1, b0s);
                 if ((s & 3u) == 3u) {
                     ya += (double)pa;
                     yb += (double)pb;
@@ -595,16 +607,10 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
                 }
             } else {
                 const float c = JUMPS ? d.eu_a0 + jl : d.eu_a0;
-                float dya = c, dyb = c;
-                if (first_neg) {
-                    va = vb = fmaxf(d.f_v0 + d.eu_b0, 0.0f);
-                } else {
-                    dya = euler_f32(d, va, sa, sb, c);
-                    dyb = euler_f32(d, vb, -sa, -sb, c);
-                }
-                ya += (double)dya;
-                yb += (double)dyb;
+                ya += (double)euler_f32(d, va, sa, sb, c, c1, b0s);
+                yb += (double)euler_f32(d, vb, -sa, -sb, c, c1, b0s);
             }
+            b0s = d.eu_b0;
         } else {
             float dya, dyb;
             qe_pair_f32<JUMPS>(d, va, vb, anti, za, zb, jl, dya, dyb);

@@ -119,3 +119,22 @@ def test_replay_fp32_path_close_to_oracle(ope, ora, scheme, lam, kind):
     assert frac_close > 0.998
     price, se = _ref_price(pay_ref, True)
     assert abs(run.result.price - price) < 0.02 * se
+
+
+@pytest.mark.parametrize("scheme,lam,kind,n_steps", [(0, None, 0, 37), (0, 0.5, 0, 1), (0, 0.5, 1, 37),
+                                                     (0, None, 2, 9), (1, 0.5, 0, 37)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_replay_negative_v0(ope, ora, scheme, lam, kind, n_steps, precision):
+    # v0 < 0: the first step sees v+ = 0 (mc.rs:245): no diffusion, and Euler's
+    # v' = max(v0 + kappa theta dt, 0) (mc.rs:254); QE starts from v+ = 0
+    hp = dict(HP, v0=-0.02)
+    pay_ref, st_ref, run = _oracle_and_gpu(ope, ora, scheme, lam, kind, True, 0, 600, n_steps, hp=hp,
+                                           precision=precision)
+    st = run.path_states
+    if precision == 0:
+        np.testing.assert_allclose(st[..., :2], st_ref[..., :2], rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(run.path_payoffs, pay_ref, rtol=0, atol=1e-12 * S0)
+    else:
+        np.testing.assert_allclose(st[..., 0], st_ref[..., 0], rtol=0, atol=2e-5)
+        np.testing.assert_allclose(st[..., 1], st_ref[..., 1], rtol=0, atol=2e-5)
+        assert (np.abs(run.path_payoffs - pay_ref) <= 5e-3).mean() > 0.998
