@@ -145,10 +145,10 @@ __device__ __forceinline__ PairStream make_stream(const McDesc& d, uint64_t pair
     return s;
 }
 
-// Jump of step s given the 16 high bits h of its Poisson uniform word:
-// jump_sum = mu_j n + sigma_j sqrt(n) z (mc.rs:139-140).  h < thr0 >> 16
-// means count 0 whatever the low bits; otherwise (probability ~ lambda dt)
-// the jump call of step s supplies the low 16 bits (word 2), completing the
+// Jump of step s given the 16 high bits h of its Poisson uniform word, for
+// h >= thr0 >> 16 (the caller's test: below it the count is 0 whatever the
+// low bits): jump_sum = mu_j n + sigma_j sqrt(n) z (mc.rs:139-140).  The
+// jump call of step s supplies the low 16 bits (word 2), completing the
 // 32-bit word W of the exact inverse-CDF count, and the jump normal (the
 // Box-Muller cosine of words 0, 1) when the count is non-zero.  FP64 mode
 // returns the jump in ln units, FP32 mode in log2 units.
@@ -158,20 +158,18 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
     using R = typename std::conditional<FP32, float, double>::type;
     R j = 0;
     n = 0;
-    if (h >= d.pois_thr0_hi) {  // rare: P ~ 1 - e^{-lambda dt} + 2^-16
-        const uint4 w = st.call(s, kTagJump);
-        const uint32_t W = (h << 16) | (w.z & 0xFFFFu);
-        if (W >= d.pois_thr0) {
-            n = poisson_tail(W, d.pois_mu, d.pois_e0);
-            if constexpr (FP32) {
-                float z, unused;
-                box_muller_f32(w.x, w.y, z, unused);
-                j = fmaf(d.f_sigma_j2 * hw_sqrt((float)n), z, d.f_mu_j2 * (float)n);
-            } else {
-                double z, unused;
-                box_muller_f64(w.x, w.y, z, unused);
-                j = d.mu_j * (double)n + d.sigma_j * sqrt((double)n) * z;
-            }
+    const uint4 w = st.call(s, kTagJump);
+    const uint32_t W = (h << 16) | (w.z & 0xFFFFu);
+    if (W >= d.pois_thr0) {
+        n = poisson_tail(W, d.pois_mu, d.pois_e0);
+        if constexpr (FP32) {
+            float z, unused;
+            box_muller_f32(w.x, w.y, z, unused);
+            j = fmaf(d.f_sigma_j2 * hw_sqrt((float)n), z, d.f_mu_j2 * (float)n);
+        } else {
+            double z, unused;
+            box_muller_f64(w.x, w.y, z, unused);
+            j = d.mu_j * (double)n + d.sigma_j * sqrt((double)n) * z;
         }
     }
     return j;
@@ -205,7 +203,12 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u;
     float jd = 0.0f;
     const float phi = MIX ? in_vgpr(d.eu_phi) : 0.0f;
-    auto one = [&](uint32_t wa, uint32_t wb, uint32_t wp, uint32_t s) {
+    // pw: the Poisson word of the step, its high half (lo false) or its low
+    // half (lo true) holding the step's 16 bits; tested without extracting
+    // the half: pw >= thr << 16 for the high half, a 16-bit compare for the
+    // low half
+    const uint32_t thr_hi = d.pois_thr0_hi, thr_hi16 = thr_hi << 16;
+    auto one = [&](uint32_t wa, uint32_t wb, uint32_t pw, bool lo, uint32_t s) {
         R za, zb;
         if constexpr (MIX) {
             const float u1 = fmaf((float)wa, 0x1p-32f, 0x1p-33f);
@@ -220,21 +223,25 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
         }
         uint32_t n = 0;
         R j = 0;
-        if constexpr (DEFER) {
-            if (wp >= d.pois_thr0_hi) {
-                if (es0 == ~0u) {
-                    es0 = s;
-                    ew0 = wp;
-                } else if (es1 == ~0u) {
-                    es1 = s;
-                    ew1 = wp;
+        if constexpr (JUMPS) {
+            const bool maybe = lo ? (uint16_t)pw >= (uint16_t)thr_hi : pw >= thr_hi16;
+            if (maybe) {  // rare: P ~ 1 - e^{-lambda dt} + 2^-16
+                const uint32_t h = lo ? (pw & 0xFFFFu) : (pw >> 16);
+                if constexpr (DEFER) {
+                    if (es0 == ~0u) {
+                        es0 = s;
+                        ew0 = h;
+                    } else if (es1 == ~0u) {
+                        es1 = s;
+                        ew1 = h;
+                    } else {
+                        uint32_t nn;
+                        jd += jump_of_step<true>(d, st, s, h, nn);
+                    }
                 } else {
-                    uint32_t nn;
-                    jd += jump_of_step<true>(d, st, s, wp, nn);
+                    j = jump_of_step<FP32>(d, st, s, h, n);
                 }
             }
-        } else if constexpr (JUMPS) {
-            j = jump_of_step<FP32>(d, st, s, wp, n);
         }
         step(za, zb, j, n, s);
     };
@@ -245,8 +252,8 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
         // half for even s, low half for odd s
         auto duo = [&](uint32_t pw, uint32_t s) {  // steps s (even) and s + 1
             const uint4 w = st.call(s >> 1, kTagNormal);
-            one(w.x, w.y, pw >> 16, s);
-            one(w.z, w.w, pw & 0xFFFFu, s + 1);
+            one(w.x, w.y, pw, false, s);
+            one(w.z, w.w, pw, true, s + 1);
         };
         for (; i + 8 <= N; i += 8) {
             const uint4 p = st.call(i >> 3, kTagPoisson);
@@ -265,19 +272,19 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
                     duo(pw[k], s);
                 } else if (s < N) {
                     const uint4 w = st.call(s >> 1, kTagNormal);
-                    one(w.x, w.y, pw[k] >> 16, s);
+                    one(w.x, w.y, pw[k], false, s);
                 }
             }
         }
     } else {
         for (; i + 2 <= N; i += 2) {
             const uint4 w = st.call(i >> 1, kTagNormal);
-            one(w.x, w.y, 0u, i);
-            one(w.z, w.w, 0u, i + 1);
+            one(w.x, w.y, 0u, false, i);
+            one(w.z, w.w, 0u, true, i + 1);
         }
         if (i < N) {
             const uint4 w = st.call(i >> 1, kTagNormal);
-            one(w.x, w.y, 0u, i);
+            one(w.x, w.y, 0u, false, i);
         }
     }
     if constexpr (DEFER) {
