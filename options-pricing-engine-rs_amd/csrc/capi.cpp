@@ -65,6 +65,26 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Pinned host buffer that only grows: one call's inputs are staged here and
+// cross PCIe in one copy, and its results come back through it.
+struct HostBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipHostFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(n, 4096);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+};
+
 // Launch groups of one call beyond the first run on these streams, so that
 // a group's last tiles overlap the next group's first ones instead of
 // leaving CUs idle between two kernels (the stream, plus these, stays
@@ -80,6 +100,8 @@ struct Context {
     hipEvent_t join[kAuxStreams] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
     DevBuf descs, tiles, strikes, slots, jobs, out;
+    DevBuf stage;           // a planned call's descriptors, tiles, strikes and jobs
+    HostBuf h_stage, h_out;
     DevBuf rz_a, rz_b, rz_j, path_pay, path_state;
     DevBuf dbg_in, dbg_out, dbg_a, dbg_b, dbg_j, dbg_n;
 };
@@ -349,12 +371,28 @@ struct RawOut {
 
 int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
     hipStream_t s = c.stream;
-    HIP_TRY(upload(c.descs, p.descs, s));
-    HIP_TRY(upload(c.tiles, p.tiles, s));
-    HIP_TRY(upload(c.strikes, p.strikes, s));
-    HIP_TRY(upload(c.jobs, p.jobs, s));
+    // descriptors, tiles, strikes and jobs: one pinned staging area, one copy
+    auto at = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t b_desc = p.descs.size() * sizeof(McDesc), b_tiles = p.tiles.size() * sizeof(uint2);
+    const size_t b_str = p.strikes.size() * sizeof(double), b_jobs = p.jobs.size() * sizeof(uint2);
+    const size_t o_tiles = at(b_desc), o_str = o_tiles + at(b_tiles), o_jobs = o_str + at(b_str);
+    const size_t total = o_jobs + b_jobs, b_out = p.n_out * sizeof(double2);
+    HIP_TRY(c.h_stage.ensure(total));
+    HIP_TRY(c.stage.ensure(total));
+    HIP_TRY(c.h_out.ensure(b_out));
+    char* h = static_cast<char*>(c.h_stage.p);
+    std::memcpy(h, p.descs.data(), b_desc);
+    std::memcpy(h + o_tiles, p.tiles.data(), b_tiles);
+    std::memcpy(h + o_str, p.strikes.data(), b_str);
+    std::memcpy(h + o_jobs, p.jobs.data(), b_jobs);
+    HIP_TRY(hipMemcpyAsync(c.stage.p, h, total, hipMemcpyHostToDevice, s));
+    char* dv = static_cast<char*>(c.stage.p);
+    const McDesc* d_desc = reinterpret_cast<const McDesc*>(dv);
+    const uint2* d_tiles = reinterpret_cast<const uint2*>(dv + o_tiles);
+    const double* d_str = reinterpret_cast<const double*>(dv + o_str);
+    const uint2* d_jobs = reinterpret_cast<const uint2*>(dv + o_jobs);
     HIP_TRY(c.slots.ensure(p.n_slots * sizeof(double2)));
-    HIP_TRY(c.out.ensure(p.n_out * sizeof(double2)));
+    HIP_TRY(c.out.ensure(b_out));
     HIP_TRY(hipEventRecord(c.ev0, s));
     // group i runs on stream i mod (1 + kAuxStreams); the auxiliary streams
     // start after the uploads (ev0) and the reduction waits for all of them.
@@ -365,10 +403,10 @@ int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
         const Plan::Group& g = p.groups[gi];
         const size_t lane = gi % (size_t)(1 + kAuxStreams);
         TileLaunch L;
-        L.descs = c.descs.as<McDesc>();
-        L.tiles = c.tiles.as<uint2>() + g.first;
+        L.descs = d_desc;
+        L.tiles = d_tiles + g.first;
         L.n_tiles = g.count;
-        L.strikes = c.strikes.as<double>();
+        L.strikes = d_str;
         L.slots = c.slots.as<double2>();
         L.fp32 = fp32;
         L.fixed = g.code < kNumCodes ? 1 : 0;
@@ -380,12 +418,12 @@ int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
         HIP_TRY(hipStreamWaitEvent(s, c.join[i], 0));
     }
     HIP_TRY(hipEventRecord(c.evk, s));
-    HIP_TRY(launch_reduce(c.descs.as<McDesc>(), c.jobs.as<uint2>(), (uint32_t)p.jobs.size(), c.slots.as<double2>(),
-                          c.out.as<double2>(), s));
+    HIP_TRY(launch_reduce(d_desc, d_jobs, (uint32_t)p.jobs.size(), c.slots.as<double2>(), c.out.as<double2>(), s));
     HIP_TRY(hipEventRecord(c.ev1, s));
-    r.sums.resize(p.n_out);
-    HIP_TRY(hipMemcpyAsync(r.sums.data(), c.out.p, p.n_out * sizeof(double2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(c.h_out.p, c.out.p, b_out, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    r.sums.resize(p.n_out);
+    std::memcpy(r.sums.data(), c.h_out.p, b_out);
     float ms = 0.0f, kms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     HIP_TRY(hipEventElapsedTime(&kms, c.ev0, c.evk));
