@@ -415,6 +415,20 @@ def time_mode(run, precision, steps, warmup):
     return prices, ses, wall / steps * 1e3, sum(kms) / len(kms)
 
 
+def host_info():
+    """The host the CPU baseline ran on: CPU model and logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
 def cpu_uneven_sample(ope, cores, budget=6e8):
     """The uneven batch's requests in order, each priced by the C restatement
     of mc.rs on `cores` threads, until `budget` path-steps have been done."""
@@ -437,7 +451,7 @@ def cpu_uneven_sample(ope, cores, budget=6e8):
     wall = time.perf_counter() - t0
     return {"value": done / wall, "unit": "path-steps/s", "cores": cores, "kind": "port",
             "sample": f"the first {n_req} of the 256 requests ({done:.3g} path-steps), one mc_price call each; "
-                      f"{cores} threads, chunk seeding as mc.rs:159-172; {wall:.2f} s wall"}
+                      f"{cores} threads, chunk seeding as mc.rs:159-172; {wall:.2f} s wall", **host_info()}
 
 
 def cpu_baseline(workload, ope=None):
@@ -465,7 +479,7 @@ def cpu_baseline(workload, ope=None):
     return {"value": n_paths * n_steps / wall, "unit": "path-steps/s", "cores": cores, "kind": "port",
             "sample": sample + f"; {cores} threads, chunk seeding as mc.rs:159-172; {wall:.2f} s wall, "
                                f"~{wall * cores:.0f} thread-seconds",
-            "price": r[0], "std_error": r[1]}
+            "price": r[0], "std_error": r[1], **host_info()}
 
 
 def load_traffic(kernel_tag):
