@@ -16,7 +16,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 REF = dict(v0=0.04, kappa=2.0, theta=0.04, sigma=0.3, rho=-0.7)   # mc.rs:351-353
-BASE = dict(v0=0.04, kappa=1.5, theta=0.04, sigma=0.5, rho=-0.7)  # BASELINE params
+BASE = dict(v0=0.04, kappa=1.5, theta=0.04, sigma=0.5, rho=-0.7)  # the MC workloads' parameters
 PRECISIONS = [0, 1]
 
 

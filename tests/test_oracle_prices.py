@@ -36,7 +36,7 @@ def lewis_price(s, k, t, r, q, v0, kappa, theta, sigma, rho, lam=0.0, mu_j=0.0, 
 
 
 PARAM_SETS = [
-    (0.04, 1.5, 0.04, 0.5, -0.7),   # BASELINE params (Feller violated)
+    (0.04, 1.5, 0.04, 0.5, -0.7),   # the MC workloads' parameters (Feller violated)
     (0.04, 2.0, 0.04, 0.3, -0.7),   # reference MC test params (mc.rs:351-353)
     (0.09, 1.0, 0.09, 0.8, -0.5),
     (0.04, 5.0, 0.04, 1.2, -0.7),   # QE bias test (mc.rs:373)
@@ -70,7 +70,9 @@ def test_bates_cf_pricer_matches_lewis(ora, k, t):
     assert abs(got - want) < 2e-6 * max(1.0, want)
 
 
-def test_survey_anchor_prices(ora):
+def test_independent_anchor_prices(ora):
+    # the MC workloads' parameters at S = K = 100, T = 1, r = 0.03: 8.8027
+    # (Heston) and 10.2700 (Bates) from an independent textbook CF pricer
     hp = ora.heston(0.04, 1.5, 0.04, 0.5, -0.7)
     assert abs(ora.heston_price(100, 100, 1, 0.03, 0, hp) - 8.8027) < 1e-3
     assert abs(ora.bates_price(100, 100, 1, 0.03, 0, hp, 0.5, -0.1, 0.15) - 10.2700) < 1e-3
@@ -179,7 +181,7 @@ def test_odd_path_count_drops_one_path(ora):
 
 
 def test_baseline_cfg1_against_analytic(ora):
-    # Heston European, Euler, FP64, 100k paths, 252 steps, seed 42 (BASELINE params)
+    # Heston European, Euler, FP64, 100k paths, 252 steps, seed 42 (the MC workloads' parameters)
     hp = ora.heston(0.04, 1.5, 0.04, 0.5, -0.7)
     price, se, _, _, count = ora.mc_price(100, 1, 0.03, 0, hp, ora.payoff(), 100_000, 252, 42, True, 0,
                                           n_threads=THREADS)
