@@ -423,13 +423,16 @@ __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z) {
 // (-za, -zb).  |za| is common to both, so the exponential branch's erfc is
 // evaluated once per pair, and only when a lane of the wave needs it.  The
 // variances are >= 0 on entry (QE's v' is never negative, and v0 is clamped
-// by the caller).  The price variance term K3 v + K4 v' has K3 = K4
-// (gamma1 = gamma2 = 1/2), so sqrt(max(K3 v + K4 v', 0)) =
-// sqrt(max(K3, 0)) sqrt(v + v'), and the constant folds into zb.  Returns the
-// log2-price increments.
+// by the caller).  Sibling b sits behind a uniform branch on `anti`:
+// computing it unconditionally issues 6 % fewer instructions (no register
+// copies for the branch) but measured 2.5 % slower.  The price variance term
+// K3 v + K4 v' has K3 = K4 (gamma1 = gamma2 = 1/2), so
+// sqrt(max(K3 v + K4 v', 0)) = sqrt(max(K3, 0)) sqrt(v + v'), and the
+// constant folds into zb.  q0 is the increment's constant term, held in a
+// vector register by the caller.  Returns the log2-price increments.
 template <bool JUMPS>
 __device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& vb, bool anti, float za, float zb,
-                                            float jl, float& dya, float& dyb) {
+                                            float jl, float q0, float& dya, float& dyb) {
     const QeSib a = qe_sib_f32(d, va, za);
     QeSib b = {0.0f, 0.0f, 0.0f, false};
     if (anti) b = qe_sib_f32(d, vb, -za);
@@ -444,7 +447,7 @@ __device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& v
     const float na = a.mr * (a.ex ? la : a.bz2);
     const float zs = d.qe_sq3 * zb;
     auto incr = [&](float vp, float vn, float z) {
-        float dy = fmaf(hw_sqrt(vp + vn), z, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, d.qe_q0)));
+        float dy = fmaf(hw_sqrt(vp + vn), z, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, q0)));
         if (JUMPS) dy += jl;
         return dy;
     };
@@ -595,6 +598,7 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     // and every later one b0.
     float b0s = d.f_v0 < 0.0f ? d.f_v0 + d.eu_b0 : d.eu_b0;
     const float c1 = in_vgpr(d.eu_c1);
+    const float q0 = in_vgpr(d.qe_q0);
     constexpr bool kMixed = Src::kPhilox && SCHEME == kSchemeEuler;
     auto step = [&](float za, float zb, float jl, uint32_t, uint32_t s) {
         if (SCHEME == kSchemeEuler) {
@@ -620,7 +624,7 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
             b0s = d.eu_b0;
         } else {
             float dya, dyb;
-            qe_pair_f32<JUMPS>(d, va, vb, anti, za, zb, jl, dya, dyb);
+            qe_pair_f32<JUMPS>(d, va, vb, anti, za, zb, jl, q0, dya, dyb);
             accumulate(ya, pa, dya, s);
             accumulate(yb, pb, dyb, s);  // dyb = 0 without the sibling
         }
