@@ -576,7 +576,8 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     const uint32_t N = d.n_steps;
 
     // European payoffs need only y_T: increments are summed in FP32 over
-    // blocks of 4 steps and folded into the FP64 total once per block (Euler:
+    // blocks of 8 steps (one per Poisson call of the Bates draws) and folded
+    // into the FP64 total once per block (Euler:
     // without the constant a0, added as N a0 after the loop, and without the
     // jumps, which both siblings share and which are summed once, in jsum).
     // Asian and barrier payoffs read y every step and fold every increment.
@@ -584,7 +585,7 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     auto accumulate = [&](double& y, float& part, float dy, uint32_t s) {
         if (PAYOFF == kPayEuropean) {
             part += dy;
-            if ((s & 3u) == 3u) {
+            if ((s & 7u) == 7u) {
                 y += (double)part;
                 part = 0.0f;
             }
@@ -611,7 +612,7 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
                 if (JUMPS) jsum += jl;
                 pa = euler_f32(d, va, sa, sb, pa, c1, b0s);
                 pb = euler_f32(d, vb, -sa, -sb, pb, c1, b0s);
-                if ((s & 3u) == 3u) {
+                if ((s & 7u) == 7u) {
                     ya += (double)pa;
                     yb += (double)pb;
                     pa = pb = 0.0f;

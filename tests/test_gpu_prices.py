@@ -145,7 +145,7 @@ def test_barrier_at_or_below_spot_pays_discounted_rebate(ope, precision):
 @pytest.mark.parametrize("lam", [0.0, 0.5, 30.0])
 def test_barrier_at_infinity_is_european(ope, precision, scheme, lam):
     # pathwise identical payoffs: bit-identical in FP64; in FP32-path mode the
-    # European kernel sums log-price increments in 4-step FP32 blocks (and,
+    # European kernel sums log-price increments in 8-step FP32 blocks (and,
     # with jumps, draws the jump sizes after the last step from two per-lane
     # slots, inline beyond two) while the barrier kernel folds every step,
     # jump included, into FP64, so only rounding differs.  lam = 30 over 30
