@@ -227,8 +227,8 @@ int make_desc(const Request& r, McDesc& d) {
         // FP32-path folding: m = m1 v + m0, s2 = s1 v + s0, log2-scaled price terms
         d.qe_m1 = (float)d.ekt;
         d.qe_m0 = (float)(hp.theta * d.one_m_ekt);
-        d.qe_s1 = (float)(hp.sigma * hp.sigma * d.ekt * d.one_m_ekt / hp.kappa);
-        d.qe_s0 = (float)d.qe_c2;
+        d.qe_s1 = (float)(0.5 * (hp.sigma * hp.sigma * d.ekt * d.one_m_ekt / hp.kappa));
+        d.qe_s0 = (float)(0.5 * d.qe_c2);
         d.qe_q0 = (float)(kLog2eD * d.qe_a0);
         d.qe_q1 = (float)(kLog2eD * d.k1);
         d.qe_q2 = (float)(kLog2eD * d.k2);

@@ -57,7 +57,7 @@ struct alignas(16) McDesc {
     float eu_r2, eu_kb, eu_phi;    // -2 ln2 a2^2, b2/a2, acos(rho)/(2 pi): mixed Box-Muller
     float f_rho, f_rhoc;           // z2 = rho za + sqrt(1-rho^2) zb
     float qe_m1, qe_m0;            // m = m1 v+ + m0
-    float qe_s1, qe_s0;            // s2 = s1 v+ + s0
+    float qe_s1, qe_s0;            // s2 / 2 = s1 v+ + s0 (halved on the host)
     float qe_q0, qe_q1, qe_q2;     // dy = q0 + q1 v+ + q2 v' + sq3 sqrt(v+ + v') zb
     float qe_sq3;                  // sqrt(max(q3, 0)), q3 = q4 the log2-scaled K3 = K4
     float pad_f_;

@@ -407,10 +407,11 @@ struct QeSib {
 
 __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z) {
     QeSib o;
-    const float m = fmaf(d.qe_m1, v, d.qe_m0), s2 = fmaf(d.qe_s1, v, d.qe_s0);
-    const float mm2 = 2.0f * (m * m);
-    o.ex = s2 > 0.75f * mm2;
-    o.t = fminf(mm2 * hw_rcp(o.ex ? fmaf(0.5f, mm2, s2) : s2), 2e12f);
+    // s2h = s2 / 2 (the host halves s1 and s0): t = m^2 / s2h or m^2 / (s2h + m^2 / 2)
+    const float m = fmaf(d.qe_m1, v, d.qe_m0), s2h = fmaf(d.qe_s1, v, d.qe_s0);
+    const float msq = m * m;
+    o.ex = s2h > 0.75f * msq;
+    o.t = fminf(msq * hw_rcp(o.ex ? fmaf(0.5f, msq, s2h) : s2h), 2e12f);
     const float a = o.t - 1.0f;
     const float b2 = a + hw_sqrt(o.t * a);
     const float bz = hw_sqrt(b2) + z;
@@ -577,20 +578,33 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
 
     // European payoffs need only y_T: increments are summed in FP32 over
     // blocks of 8 steps (one per Poisson call of the Bates draws) and folded
-    // into the FP64 total once per block (Euler:
-    // without the constant a0, added as N a0 after the loop, and without the
-    // jumps, which both siblings share and which are summed once, in jsum).
-    // Asian and barrier payoffs read y every step and fold every increment.
+    // into the FP64 total once per block (Euler: without the constant a0,
+    // added as N a0 after the loop, and without the jumps, which both
+    // siblings share and which are summed once, in jsum).  Asian payoffs
+    // read y = ya + pa every step, ya the FP64 total of the finished blocks
+    // and pa the FP32 sum of the current one, through fa = (float)ya: S is
+    // exp2(fa + pa), one FP32 rounding of y as before the blocks.  Barrier
+    // payoffs fold every increment into the FP64 y, whose maximum the
+    // knock-out test reads.
     float pa = 0.0f, pb = 0.0f, jsum = 0.0f;
+    float fa = 0.0f, fb = 0.0f;
+    constexpr bool kBlocks = PAYOFF == kPayEuropean || PAYOFF == kPayAsian;
     auto accumulate = [&](double& y, float& part, float dy, uint32_t s) {
-        if (PAYOFF == kPayEuropean) {
+        if (kBlocks) {
             part += dy;
-            if ((s & 7u) == 7u) {
-                y += (double)part;
-                part = 0.0f;
-            }
         } else {
             y += (double)dy;
+        }
+    };
+    auto fold = [&](uint32_t s) {
+        if (kBlocks && (s & 7u) == 7u) {
+            ya += (double)pa;
+            yb += (double)pb;
+            pa = pb = 0.0f;
+            if (PAYOFF == kPayAsian) {
+                fa = (float)ya;
+                fb = (float)yb;
+            }
         }
     };
     // A negative v0 has v+ = 0 in the first Euler step (mc.rs:245-254): no
@@ -612,11 +626,10 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
                 if (JUMPS) jsum += jl;
                 pa = euler_f32(d, va, sa, sb, pa, c1, b0s);
                 pb = euler_f32(d, vb, -sa, -sb, pb, c1, b0s);
-                if ((s & 7u) == 7u) {
-                    ya += (double)pa;
-                    yb += (double)pb;
-                    pa = pb = 0.0f;
-                }
+            } else if (PAYOFF == kPayAsian) {
+                const float c = JUMPS ? d.eu_a0 + jl : d.eu_a0;
+                pa += euler_f32(d, va, sa, sb, c, c1, b0s);
+                pb += euler_f32(d, vb, -sa, -sb, c, c1, b0s);
             } else {
                 const float c = JUMPS ? d.eu_a0 + jl : d.eu_a0;
                 ya += (double)euler_f32(d, va, sa, sb, c, c1, b0s);
@@ -632,9 +645,10 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
         // sibling b's running state is updated unconditionally (unused without
         // the sibling): see sim_pair_f64
         if (PAYOFF == kPayAsian) {
-            suma += hw_exp2((float)ya);
-            sumb += hw_exp2((float)yb);
+            suma += hw_exp2(fa + pa);
+            sumb += hw_exp2(fb + pb);
         }
+        fold(s);
         if (PAYOFF == kPayUpAndOut) {
             mxa = fmax(mxa, ya);
             mxb = fmax(mxb, yb);
@@ -650,6 +664,10 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
             src.get(i, za, zb, j);
             step((float)za, (float)zb, (float)(j * kLog2e), 0u, i);
         }
+    }
+    if (PAYOFF == kPayAsian) {  // the last partial block
+        ya += (double)pa;
+        yb += (double)pb;
     }
     if (PAYOFF == kPayEuropean) {
         // the last partial block, the Euler constant N a0 and the jumps
