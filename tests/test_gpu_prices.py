@@ -188,3 +188,21 @@ def test_reference_api_shapes(ope):
     r2 = ope.mc_bates(100.0, 1.0, 0.05, 0.0, hp, 0.5, -0.10, 0.15, ope.Payoff.UpAndOut(100.0, 130.0, 0.0), cfg)
     assert 0 < r1.price < 10 and r1.std_error > 0
     assert 0 < r2.price < 10 and r2.std_error > 0
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_long_paths_and_ragged_counts(ope, scheme):
+    # 4096 steps (dt = 2.4e-4): the draw loops' blocks of 8 steps, the
+    # 8-step FP32 partial sums and the FP64 fold over a long path; an odd path
+    # count just above the 4-pairs-per-thread threshold (2^20 pairs) drops
+    # its last path (mc.rs:160-161) and ends in a ragged tile.  FP32 and FP64
+    # see the same draws, so they agree to a small fraction of the SE.
+    hp = ope.HestonParams(**BASE)
+    for n_paths, n_steps in ((20_001, 4096), (2 * (1 << 20) + 2 * 37 + 1, 13)):
+        cfg = ope.McConfig(n_paths=n_paths, n_steps=n_steps, seed=77, scheme=scheme)
+        for pay in (ope.Payoff.European(100.0), ope.Payoff.AsianArithmetic(100.0)):
+            r64 = run(ope, 0, pay, cfg, hp=BASE, jumps=(0.5, -0.1, 0.15))
+            r32 = run(ope, 1, pay, cfg, hp=BASE, jumps=(0.5, -0.1, 0.15))
+            assert r64.stats.count == (n_paths // 2) * 2
+            assert r64.stats.path_steps == (n_paths // 2) * 2 * n_steps
+            assert abs(r32.result.price - r64.result.price) <= 0.1 * r64.result.std_error, (n_paths, n_steps, pay)
