@@ -175,6 +175,10 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
     return j;
 }
 
+struct NoBlockEnd {
+    __device__ void operator()(uint32_t) const {}
+};
+
 // Drives the Philox-mode draws of one pair through all steps, calling
 // step(za, zb, jump, n, s) once per step s.  Heston: one Philox call per two
 // steps.  Bates: the same normals plus one Poisson call per eight steps (16
@@ -189,14 +193,20 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
 // its lanes jumps.  A lane's third and later events are drawn at once.  The
 // draws are the same words either way.
 //
+// block_end(s) is called after step s for every s = 8k + 7 of the steps
+// that complete a block of eight (FP32 mode, in which the step sums its
+// increments in FP32 over such blocks); the loops run whole blocks, so the
+// call sits at the end of a loop body rather than behind a per-step branch,
+// whose two sides would need register copies.
+//
 // MIX (FP32 Euler): step() gets the scaled, correlated normals of the Euler
 // step, sa = a2 za and sb = b2 (rho za + sqrt(1 - rho^2) zb), instead of
 // (za, zb).  With za = r cos t and zb = r sin t, rho = cos phi and
 // sqrt(1 - rho^2) = sin phi, the mixed normal is r cos(t - phi): one cosine
 // replaces the sine and the mixing products, and a2, b2 fold into r.
-template <bool FP32, bool JUMPS, bool DEFER = false, bool MIX = false, class Step>
+template <bool FP32, bool JUMPS, bool DEFER = false, bool MIX = false, class Step, class BlockEnd = NoBlockEnd>
 __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& st, Step&& step,
-                                             float* jsum = nullptr) {
+                                             float* jsum = nullptr, BlockEnd block_end = BlockEnd{}) {
     using R = typename std::conditional<FP32, float, double>::type;
     static_assert(!DEFER || (FP32 && JUMPS), "deferred jumps are an FP32 jump-model mode");
     static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
@@ -208,7 +218,7 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     // the half: pw >= thr << 16 for the high half, a 16-bit compare for the
     // low half
     const uint32_t thr_hi = d.pois_thr0_hi, thr_hi16 = thr_hi << 16;
-    auto one = [&](uint32_t wa, uint32_t wb, uint32_t pw, bool lo, uint32_t s) {
+    auto one = [&](uint32_t wa, uint32_t wb, uint32_t pw, bool lo, uint32_t s, bool end = false) {
         R za, zb;
         if constexpr (MIX) {
             const float u1 = fmaf((float)wa, 0x1p-32f, 0x1p-33f);
@@ -244,23 +254,24 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
             }
         }
         step(za, zb, j, n, s);
+        if (end) block_end(s);
     };
     const uint32_t N = d.n_steps;
     uint32_t i = 0;
     if constexpr (JUMPS) {
         // Poisson halves of step s: word (s >> 1) & 3 of call s >> 3, high
         // half for even s, low half for odd s
-        auto duo = [&](uint32_t pw, uint32_t s) {  // steps s (even) and s + 1
+        auto duo = [&](uint32_t pw, uint32_t s, bool end = false) {  // steps s (even) and s + 1
             const uint4 w = st.call(s >> 1, kTagNormal);
             one(w.x, w.y, pw, false, s);
-            one(w.z, w.w, pw, true, s + 1);
+            one(w.z, w.w, pw, true, s + 1, end);
         };
         for (; i + 8 <= N; i += 8) {
             const uint4 p = st.call(i >> 3, kTagPoisson);
             duo(p.x, i);
             duo(p.y, i + 2);
             duo(p.z, i + 4);
-            duo(p.w, i + 6);
+            duo(p.w, i + 6, true);
         }
         if (i < N) {  // 1..7 steps left
             const uint4 p = st.call(i >> 3, kTagPoisson);
@@ -277,6 +288,29 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
             }
         }
     } else {
+        if constexpr (FP32) {
+            // blocks of eight steps; FP32 Euler unrolls a block (the
+            // scheduler overlaps consecutive steps), the QE step, three times
+            // longer, measured slower unrolled and keeps a two-step body
+            for (; i + 8 <= N; i += 8) {
+                if constexpr (MIX) {
+#pragma unroll
+                    for (uint32_t k = 0; k < 8; k += 2) {
+                        const uint4 w = st.call((i + k) >> 1, kTagNormal);
+                        one(w.x, w.y, 0u, false, i + k);
+                        one(w.z, w.w, 0u, true, i + k + 1);
+                    }
+                } else {
+#pragma nounroll
+                    for (uint32_t k = 0; k < 8; k += 2) {
+                        const uint4 w = st.call((i + k) >> 1, kTagNormal);
+                        one(w.x, w.y, 0u, false, i + k);
+                        one(w.z, w.w, 0u, true, i + k + 1);
+                    }
+                }
+                block_end(i + 7);
+            }
+        }
         for (; i + 2 <= N; i += 2) {
             const uint4 w = st.call(i >> 1, kTagNormal);
             one(w.x, w.y, 0u, false, i);
@@ -405,10 +439,10 @@ struct QeSib {
     bool ex;
 };
 
-__device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z) {
+__device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z, float m0, float s0) {
     QeSib o;
     // s2h = s2 / 2 (the host halves s1 and s0): t = m^2 / s2h or m^2 / (s2h + m^2 / 2)
-    const float m = fmaf(d.qe_m1, v, d.qe_m0), s2h = fmaf(d.qe_s1, v, d.qe_s0);
+    const float m = fmaf(d.qe_m1, v, m0), s2h = fmaf(d.qe_s1, v, s0);
     const float msq = m * m;
     o.ex = s2h > 0.75f * msq;
     o.t = fminf(msq * hw_rcp(o.ex ? fmaf(0.5f, msq, s2h) : s2h), 2e12f);
@@ -421,31 +455,38 @@ __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z) {
 }
 
 // One QE step of both siblings of a pair: sibling a draws (za, zb), sibling b
-// (-za, -zb).  |za| is common to both, so the exponential branch's erfc is
-// evaluated once per pair, and only when a lane of the wave needs it.  The
-// variances are >= 0 on entry (QE's v' is never negative, and v0 is clamped
-// by the caller).  Sibling b sits behind a uniform branch on `anti`:
-// computing it unconditionally issues 6 % fewer instructions (no register
-// copies for the branch) but measured 2.5 % slower.  The price variance term
-// K3 v + K4 v' has K3 = K4 (gamma1 = gamma2 = 1/2), so
-// sqrt(max(K3 v + K4 v', 0)) = sqrt(max(K3, 0)) sqrt(v + v'), and the
-// constant folds into zb.  q0 is the increment's constant term, held in a
-// vector register by the caller.  Returns the log2-price increments.
-template <bool JUMPS>
-__device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& vb, bool anti, float za, float zb,
-                                            float jl, float q0, float& dya, float& dyb) {
-    const QeSib a = qe_sib_f32(d, va, za);
-    QeSib b = {0.0f, 0.0f, 0.0f, false};
-    if (anti) b = qe_sib_f32(d, vb, -za);
-    float la = 0.0f, lb = 0.0f;
-    if (a.ex || b.ex) {
+// (-za, -zb), when the pair has one (ANTI, fixed per compiled loop: a
+// runtime flag costs register copies and selects at every step).  |za| is
+// common to both, so the exponential branch's erfc is evaluated once per
+// pair, and only when a lane of the wave needs it; its results replace the
+// quadratic branch's factors inside that branch, so that nothing has to be
+// initialised on the way round it.
+// The variances are >= 0 on entry (QE's v' is never negative, and v0 is
+// clamped by the caller).  The price variance term K3 v + K4 v' has K3 = K4
+// (gamma1 = gamma2 = 1/2), so sqrt(max(K3 v + K4 v', 0)) = sqrt(max(K3, 0))
+// sqrt(v + v'), and the constant folds into zb.  m0, s0 and q0 are uniform
+// constants held in vector registers by the caller.  Returns the log2-price
+// increments.
+template <bool JUMPS, bool ANTI>
+__device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& vb, float za, float zb, float jl,
+                                            float m0, float s0, float q0, float& dya, float& dyb) {
+    const QeSib a = qe_sib_f32(d, va, za, m0, s0);
+    QeSib b{};
+    if (ANTI) b = qe_sib_f32(d, vb, -za, m0, s0);
+    // the factor of m r: (b + z)^2, or ln(t / (1 - ncdf(z))) on exponential lanes
+    float qa = a.bz2, qb = b.bz2;
+    if (a.ex || (ANTI && b.ex)) {
         const float e = ncdf_neg_abs_f32(za), ome = 1.0f - e;
         const bool pos = za >= 0.0f;
-        // ln(t / (1 - ncdf(z))) for z = za (sibling a) and z = -za (sibling b)
-        la = fmaxf(hw_log2(a.t * hw_rcp(pos ? e : ome)), 0.0f) * 0.69314718055994531f;
-        lb = fmaxf(hw_log2(b.t * hw_rcp(pos ? ome : e)), 0.0f) * 0.69314718055994531f;
+        // z = za for sibling a, z = -za for sibling b
+        const float la = fmaxf(hw_log2(a.t * hw_rcp(pos ? e : ome)), 0.0f) * 0.69314718055994531f;
+        qa = a.ex ? la : qa;
+        if (ANTI) {
+            const float lb = fmaxf(hw_log2(b.t * hw_rcp(pos ? ome : e)), 0.0f) * 0.69314718055994531f;
+            qb = b.ex ? lb : qb;
+        }
     }
-    const float na = a.mr * (a.ex ? la : a.bz2);
+    const float na = a.mr * qa;
     const float zs = d.qe_sq3 * zb;
     auto incr = [&](float vp, float vn, float z) {
         float dy = fmaf(hw_sqrt(vp + vn), z, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, q0)));
@@ -454,8 +495,8 @@ __device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& v
     };
     dya = incr(va, na, zs);
     va = na;
-    if (anti) {
-        const float nb = b.mr * (b.ex ? lb : b.bz2);
+    if (ANTI) {
+        const float nb = b.mr * qb;
         dyb = incr(vb, nb, -zs);
         vb = nb;
     } else {
@@ -565,9 +606,12 @@ __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bo
 }
 
 // --------------------------------------------- pair simulation, FP32 path
-template <int SCHEME, bool JUMPS, int PAYOFF, class Src>
-__device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bool anti, PathState& ea,
-                                             PathState& eb) {
+// ANTI: the pair has a sibling (QE's loop is compiled once per case; the
+// Euler loop always computes the sibling, and `anti` decides whether its
+// result is kept).
+template <int SCHEME, bool JUMPS, int PAYOFF, bool ANTI, class Src>
+__device__ __forceinline__ void sim_pair_f32_loop(const McDesc& d, const Src& src, bool anti, PathState& ea,
+                                                  PathState& eb) {
     double ya = 0.0, yb = 0.0;
     // QE only ever reads v+ = max(v, 0) and never produces a negative v; the
     // Euler step's handling of a negative v0 is below
@@ -576,35 +620,31 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
     const uint32_t N = d.n_steps;
 
-    // European payoffs need only y_T: increments are summed in FP32 over
-    // blocks of 8 steps (one per Poisson call of the Bates draws) and folded
-    // into the FP64 total once per block (Euler: without the constant a0,
-    // added as N a0 after the loop, and without the jumps, which both
-    // siblings share and which are summed once, in jsum).  Asian payoffs
-    // read y = ya + pa every step, ya the FP64 total of the finished blocks
-    // and pa the FP32 sum of the current one, through fa = (float)ya: S is
-    // exp2(fa + pa), one FP32 rounding of y as before the blocks.  Barrier
-    // payoffs fold every increment into the FP64 y, whose maximum the
-    // knock-out test reads.
+    // Increments are summed in FP32 over blocks of 8 steps (one per Poisson
+    // call of the Bates draws), pa / pb, and folded into the FP64 totals
+    // ya / yb once per block (block_end of the step driver).  European
+    // payoffs need only y_T (Euler: the partial sums leave out the constant
+    // a0, added as N a0 after the loop, and the jumps, which both siblings
+    // share and which are summed once, in jsum).  Asian payoffs read y =
+    // ya + pa every step through fa = (float)ya: S is exp2(fa + pa).
+    // Barrier payoffs need the maximum of y over the steps: the block keeps
+    // the maximum of its partial sums, ma / mb, and the fold takes ya +
+    // (double)ma, the block's highest y, into the FP64 maxima mxa / mxb.
     float pa = 0.0f, pb = 0.0f, jsum = 0.0f;
     float fa = 0.0f, fb = 0.0f;
-    constexpr bool kBlocks = PAYOFF == kPayEuropean || PAYOFF == kPayAsian;
-    auto accumulate = [&](double& y, float& part, float dy, uint32_t s) {
-        if (kBlocks) {
-            part += dy;
-        } else {
-            y += (double)dy;
+    float ma = -__builtin_huge_valf(), mb = -__builtin_huge_valf();
+    auto fold = [&](uint32_t) {
+        if (PAYOFF == kPayUpAndOut) {
+            mxa = fmax(mxa, ya + (double)ma);
+            mxb = fmax(mxb, yb + (double)mb);
+            ma = mb = -__builtin_huge_valf();
         }
-    };
-    auto fold = [&](uint32_t s) {
-        if (kBlocks && (s & 7u) == 7u) {
-            ya += (double)pa;
-            yb += (double)pb;
-            pa = pb = 0.0f;
-            if (PAYOFF == kPayAsian) {
-                fa = (float)ya;
-                fb = (float)yb;
-            }
+        ya += (double)pa;
+        yb += (double)pb;
+        pa = pb = 0.0f;
+        if (PAYOFF == kPayAsian) {
+            fa = (float)ya;
+            fb = (float)yb;
         }
     };
     // A negative v0 has v+ = 0 in the first Euler step (mc.rs:245-254): no
@@ -613,7 +653,7 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     // and every later one b0.
     float b0s = d.f_v0 < 0.0f ? d.f_v0 + d.eu_b0 : d.eu_b0;
     const float c1 = in_vgpr(d.eu_c1);
-    const float q0 = in_vgpr(d.qe_q0);
+    const float q0 = in_vgpr(d.qe_q0), m0 = in_vgpr(d.qe_m0), s0 = in_vgpr(d.qe_s0);
     constexpr bool kMixed = Src::kPhilox && SCHEME == kSchemeEuler;
     auto step = [&](float za, float zb, float jl, uint32_t, uint32_t s) {
         if (SCHEME == kSchemeEuler) {
@@ -626,21 +666,17 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
                 if (JUMPS) jsum += jl;
                 pa = euler_f32(d, va, sa, sb, pa, c1, b0s);
                 pb = euler_f32(d, vb, -sa, -sb, pb, c1, b0s);
-            } else if (PAYOFF == kPayAsian) {
+            } else {
                 const float c = JUMPS ? d.eu_a0 + jl : d.eu_a0;
                 pa += euler_f32(d, va, sa, sb, c, c1, b0s);
                 pb += euler_f32(d, vb, -sa, -sb, c, c1, b0s);
-            } else {
-                const float c = JUMPS ? d.eu_a0 + jl : d.eu_a0;
-                ya += (double)euler_f32(d, va, sa, sb, c, c1, b0s);
-                yb += (double)euler_f32(d, vb, -sa, -sb, c, c1, b0s);
             }
             b0s = d.eu_b0;
         } else {
             float dya, dyb;
-            qe_pair_f32<JUMPS>(d, va, vb, anti, za, zb, jl, q0, dya, dyb);
-            accumulate(ya, pa, dya, s);
-            accumulate(yb, pb, dyb, s);  // dyb = 0 without the sibling
+            qe_pair_f32<JUMPS, ANTI>(d, va, vb, za, zb, jl, m0, s0, q0, dya, dyb);
+            pa += dya;
+            pb += dyb;  // dyb = 0 without the sibling
         }
         // sibling b's running state is updated unconditionally (unused without
         // the sibling): see sim_pair_f64
@@ -648,34 +684,31 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
             suma += hw_exp2(fa + pa);
             sumb += hw_exp2(fb + pb);
         }
-        fold(s);
         if (PAYOFF == kPayUpAndOut) {
-            mxa = fmax(mxa, ya);
-            mxb = fmax(mxb, yb);
+            ma = fmaxf(ma, pa);
+            mb = fmaxf(mb, pb);
         }
     };
 
     if constexpr (Src::kPhilox) {
         constexpr bool kDefer = JUMPS && PAYOFF == kPayEuropean;
-        philox_steps<true, JUMPS, kDefer, kMixed>(d, src.st, step, &jsum);
+        philox_steps<true, JUMPS, kDefer, kMixed>(d, src.st, step, &jsum, fold);
     } else {
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
             src.get(i, za, zb, j);
             step((float)za, (float)zb, (float)(j * kLog2e), 0u, i);
+            if ((i & 7u) == 7u) fold(i);
         }
     }
-    if (PAYOFF == kPayAsian) {  // the last partial block
-        ya += (double)pa;
-        yb += (double)pb;
-    }
+    fold(N);  // the last partial block (empty when 8 divides N: pa = 0, ma = -inf)
     if (PAYOFF == kPayEuropean) {
-        // the last partial block, the Euler constant N a0 and the jumps
-        // (Euler: summed in the steps; QE: only those drawn after the loop)
+        // the Euler constant N a0 and the jumps (Euler: summed in the steps;
+        // QE: only those drawn after the loop)
         double c = (double)jsum;
         if (SCHEME == kSchemeEuler) c += (double)N * (double)d.eu_a0;
-        ya += (double)pa + c;
-        yb += (double)pb + c;
+        ya += c;
+        yb += c;
     }
 
     const double nan = __builtin_nan("");
@@ -698,6 +731,15 @@ __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bo
     };
     finish(ea, ya, va, suma, mxa);
     if (anti) finish(eb, yb, vb, sumb, mxb);
+}
+
+template <int SCHEME, bool JUMPS, int PAYOFF, class Src>
+__device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bool anti, PathState& ea,
+                                             PathState& eb) {
+    if (SCHEME == kSchemeQE && !anti)
+        sim_pair_f32_loop<SCHEME, JUMPS, PAYOFF, false>(d, src, false, ea, eb);
+    else
+        sim_pair_f32_loop<SCHEME, JUMPS, PAYOFF, true>(d, src, anti, ea, eb);
 }
 
 // ------------------------------------------------------------ reductions

@@ -206,3 +206,20 @@ def test_long_paths_and_ragged_counts(ope, scheme):
             assert r64.stats.count == (n_paths // 2) * 2
             assert r64.stats.path_steps == (n_paths // 2) * 2 * n_steps
             assert abs(r32.result.price - r64.result.price) <= 0.1 * r64.result.std_error, (n_paths, n_steps, pay)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+@pytest.mark.parametrize("jumps", [None, (0.5, -0.1, 0.15)])
+def test_fp32_without_sibling_tracks_fp64(ope, scheme, jumps):
+    # antithetic = false: one path per Philox pair (the FP32 QE loop is a
+    # separate compiled case without the sibling); the FP64 kernel draws the
+    # same words, so the two agree to a small fraction of the SE, and the
+    # path count is n_paths itself, odd counts included
+    pays = (ope.Payoff.European(100.0), ope.Payoff.AsianArithmetic(100.0), ope.Payoff.UpAndOut(100.0, 130.0))
+    for n_paths, n_steps in ((40_001, 52), (300_000, 9)):
+        cfg = ope.McConfig(n_paths=n_paths, n_steps=n_steps, seed=31, antithetic=False, scheme=scheme)
+        for pay in pays:
+            r64 = run(ope, 0, pay, cfg, hp=BASE, jumps=jumps)
+            r32 = run(ope, 1, pay, cfg, hp=BASE, jumps=jumps)
+            assert r32.stats.count == r64.stats.count == n_paths
+            assert abs(r32.result.price - r64.result.price) <= 0.1 * r64.result.std_error, (n_paths, pay)

@@ -121,6 +121,21 @@ def test_replay_fp32_path_close_to_oracle(ope, ora, scheme, lam, kind):
     assert abs(run.result.price - price) < 0.02 * se
 
 
+@pytest.mark.parametrize("scheme,lam,kind", [(1, None, 0), (1, None, 1), (1, 0.5, 2), (0, None, 1), (0, 0.5, 2)])
+def test_replay_fp32_path_without_sibling(ope, ora, scheme, lam, kind):
+    # antithetic = false: one path per draw column; the FP32 QE loop runs
+    # without the sibling's code (a separately compiled case)
+    pay_ref, st_ref, run = _oracle_and_gpu(ope, ora, scheme, lam, kind, False, 1, 2000, 50, precision=1)
+    st = run.path_states[:, :1]
+    np.testing.assert_allclose(st[..., 0], st_ref[:, :1, 0], rtol=0, atol=2e-5)
+    np.testing.assert_allclose(st[..., 1], st_ref[:, :1, 1], rtol=0, atol=2e-5)
+    assert np.isnan(run.path_states[:, 1]).all() and np.isnan(run.path_payoffs[:, 1]).all()
+    d = np.abs(run.path_payoffs[:, 0] - pay_ref[:, 0])
+    assert (d <= 5e-3).mean() > 0.998
+    price, se = _ref_price(pay_ref, False)
+    assert abs(run.result.price - price) < 0.02 * se
+
+
 @pytest.mark.parametrize("scheme,lam,kind,n_steps", [(0, None, 0, 37), (0, 0.5, 0, 1), (0, 0.5, 1, 37),
                                                      (0, None, 2, 9), (1, 0.5, 0, 37)])
 @pytest.mark.parametrize("precision", [0, 1])
