@@ -28,8 +28,8 @@ ln = 3 (hypot, ln, atan2):
 giving W_CF = 185.5 (Heston), + 57 for the jump factor and its product =
 242.5 (Bates).  An integral with p panels evaluated 15 (2p - 1) nodes.
 
-MI355X vector peaks (datasheet): FP32 157.3 TFLOP/s (packed FP32 counted),
-FP64 78.6 TFLOP/s; HBM3E 8 TB/s.
+MI355X vector peaks (datasheet): FP32 157.3 TFLOP/s (256 CUs x 4 SIMDs x 32
+FMA lanes per clock x 2 x 2.4 GHz), FP64 78.6 TFLOP/s; HBM3E 8 TB/s.
 """
 
 W_STEP = {("euler", False): 31.0, ("euler", True): 32.5, ("qe", False): 46.0, ("qe", True): 47.5}
