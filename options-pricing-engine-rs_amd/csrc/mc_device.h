@@ -446,11 +446,11 @@ __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z, f
     const float msq = m * m;
     o.ex = s2h > 0.75f * msq;
     o.t = fminf(msq * hw_rcp(o.ex ? fmaf(0.5f, msq, s2h) : s2h), 2e12f);
-    const float a = o.t - 1.0f;
-    const float b2 = a + hw_sqrt(o.t * a);
-    const float bz = hw_sqrt(b2) + z;
+    // 1 + b^2 = t + sqrt(t^2 - t), b^2 = (1 + b^2) - 1
+    const float opb = o.t + hw_sqrt(fmaf(o.t, o.t, -o.t));
+    const float bz = hw_sqrt(opb - 1.0f) + z;
     o.bz2 = bz * bz;
-    o.mr = m * hw_rcp(o.ex ? o.t : 1.0f + b2);
+    o.mr = m * hw_rcp(o.ex ? o.t : opb);
     return o;
 }
 
@@ -478,12 +478,18 @@ __device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& v
     if (a.ex || (ANTI && b.ex)) {
         const float e = ncdf_neg_abs_f32(za), ome = 1.0f - e;
         const bool pos = za >= 0.0f;
-        // z = za for sibling a, z = -za for sibling b
-        const float la = fmaxf(hw_log2(a.t * hw_rcp(pos ? e : ome)), 0.0f) * 0.69314718055994531f;
-        qa = a.ex ? la : qa;
+        // z = za for sibling a, z = -za for sibling b; with the sibling, one
+        // reciprocal serves both: 1/e = (1 - e) r and 1/(1 - e) = e r for
+        // r = 1/(e (1 - e)), e in (0, 1/2]
         if (ANTI) {
-            const float lb = fmaxf(hw_log2(b.t * hw_rcp(pos ? ome : e)), 0.0f) * 0.69314718055994531f;
+            const float r = hw_rcp(e * ome);
+            const float la = fmaxf(hw_log2(a.t * ((pos ? ome : e) * r)), 0.0f) * 0.69314718055994531f;
+            const float lb = fmaxf(hw_log2(b.t * ((pos ? e : ome) * r)), 0.0f) * 0.69314718055994531f;
+            qa = a.ex ? la : qa;
             qb = b.ex ? lb : qb;
+        } else {
+            const float la = fmaxf(hw_log2(a.t * hw_rcp(pos ? e : ome)), 0.0f) * 0.69314718055994531f;
+            qa = a.ex ? la : qa;
         }
     }
     const float na = a.mr * qa;

@@ -223,3 +223,28 @@ def test_fp32_without_sibling_tracks_fp64(ope, scheme, jumps):
             r32 = run(ope, 1, pay, cfg, hp=BASE, jumps=jumps)
             assert r32.stats.count == r64.stats.count == n_paths
             assert abs(r32.result.price - r64.result.price) <= 0.1 * r64.result.std_error, (n_paths, pay)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_zero_variance_is_deterministic(ope, precision, scheme):
+    # v0 = theta = 0: v stays 0 in both schemes (Euler: max(0 + 0, 0); QE:
+    # m = s^2 = 0, the quadratic branch's a = m / (1 + b^2) = 0), so every
+    # path is the forward S0 e^{(r - q) t} and the prices have closed forms
+    # with SE 0: the European call e^{-rT} (F_T - K), the Asian call the
+    # discounted mean of the forwards over the N monitoring dates minus K,
+    # and the up-and-out call knocks out iff some forward reaches B
+    s0, t, r, q, n = 100.0, 1.0, 0.05, 0.01, 64
+    hp = dict(v0=0.0, kappa=1.5, theta=0.0, sigma=0.5, rho=-0.7)
+    cfg = ope.McConfig(n_paths=4_000, n_steps=n, seed=3, scheme=scheme)
+    fwd = s0 * np.exp((r - q) * t * np.arange(1, n + 1) / n)
+    disc = math.exp(-r * t)
+    tol = 1e-12 * s0 if precision == 0 else 2e-5 * s0
+    cases = [(ope.Payoff.European(90.0), disc * (fwd[-1] - 90.0)),
+             (ope.Payoff.AsianArithmetic(90.0), disc * (fwd.mean() - 90.0)),
+             (ope.Payoff.UpAndOut(90.0, 103.0, 1.5), disc * 1.5),           # forward crosses 103 before T
+             (ope.Payoff.UpAndOut(90.0, 110.0, 1.5), disc * (fwd[-1] - 90.0))]
+    for pay, want in cases:
+        res = run(ope, precision, pay, cfg, hp=hp, spot=s0, expiry=t, rate=r, q=q).result
+        assert abs(res.price - want) <= tol, (pay, res.price, want)
+        assert res.std_error <= 1e-6 * s0   # zero up to the rounding of sum and sum of squares
