@@ -248,3 +248,36 @@ def test_zero_variance_is_deterministic(ope, precision, scheme):
         res = run(ope, precision, pay, cfg, hp=hp, spot=s0, expiry=t, rate=r, q=q).result
         assert abs(res.price - want) <= tol, (pay, res.price, want)
         assert res.std_error <= 1e-6 * s0   # zero up to the rounding of sum and sum of squares
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("jumps", [None, (0.5, -0.1, 0.15)])
+def test_constant_variance_is_black_scholes_or_merton(ope, ora, precision, jumps):
+    # sigma = 0 with v0 = theta: Euler keeps v = theta, so ln S is exact GBM
+    # (plus compound-Poisson jumps): European prices are Black-Scholes, or the
+    # Merton (1976) series with jumps, within the MC error, for every strike
+    hp = dict(v0=0.04, kappa=1.5, theta=0.04, sigma=0.0, rho=-0.7)
+    cfg = ope.McConfig(n_paths=400_000, n_steps=50, seed=19, scheme=0)
+    for k in (80.0, 100.0, 120.0):
+        res = run(ope, precision, ope.Payoff.European(k), cfg, hp=hp, jumps=jumps, rate=0.03).result
+        if jumps is None:
+            want = ora.bsm_price(100.0, k, 1.0, 0.03, 0.0, 0.2)
+        else:
+            want = ora.merton_price(100.0, k, 1.0, 0.03, 0.0, 0.04, *jumps)
+        assert abs(res.price - want) < 4 * res.std_error, (k, res.price, want, res.std_error)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_one_step_asian_is_european_pathwise(ope, precision, scheme):
+    # with one monitoring date the Asian average is S_T: same paths, same
+    # payoffs (FP64: the same bits; FP32 path: the Asian sum rounds S_T once
+    # more in FP32, and the European path sums its jumps after the loop)
+    cfg = ope.McConfig(n_paths=100_000, n_steps=1, seed=23, scheme=scheme)
+    e = run(ope, precision, ope.Payoff.European(100.0), cfg, hp=BASE, jumps=(0.5, -0.1, 0.15)).result
+    a = run(ope, precision, ope.Payoff.AsianArithmetic(100.0), cfg, hp=BASE, jumps=(0.5, -0.1, 0.15)).result
+    if precision == 0:
+        assert a == e
+    else:
+        # each S_T within a few FP32 roundings (1e-7 relative) of the other
+        assert abs(a.price - e.price) <= 1e-5 * e.price and abs(a.std_error - e.std_error) <= 1e-4 * e.std_error
