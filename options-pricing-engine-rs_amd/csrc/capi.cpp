@@ -70,6 +70,7 @@ struct DevBuf {
 struct HostBuf {
     void* p = nullptr;
     size_t cap = 0;
+    unsigned flags = hipHostMallocDefault;
     hipError_t ensure(size_t n) {
         if (n <= cap) return hipSuccess;
         if (p) {
@@ -79,7 +80,7 @@ struct HostBuf {
             cap = 0;
         }
         size_t want = std::max<size_t>(n, 4096);
-        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, want, flags);
         if (e == hipSuccess) cap = want;
         return e;
     }
@@ -97,11 +98,15 @@ struct Context {
     int init_status = OPE_OK;
     hipStream_t stream = nullptr;
     hipStream_t aux[kAuxStreams] = {};
-    hipEvent_t join[kAuxStreams] = {};
+    hipEvent_t join[1 + kAuxStreams] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
     DevBuf descs, tiles, strikes, slots, jobs, out;
     DevBuf stage;           // a planned call's descriptors, tiles, strikes and jobs
-    HostBuf h_stage, h_out;
+    HostBuf h_stage;
+    // the reduction kernel's results, written by the device straight into
+    // host memory (coherent, so a synchronised stream has made them visible):
+    // no device-to-host copy per call
+    HostBuf h_out{nullptr, 0, hipHostMallocCoherent};
     DevBuf rz_a, rz_b, rz_j, path_pay, path_state;
     DevBuf dbg_in, dbg_out, dbg_a, dbg_b, dbg_j, dbg_n;
 };
@@ -123,8 +128,8 @@ int ensure_init(Context& c) {
         c.init_status = fail(OPE_ERR_NO_DEVICE, "HIP initialisation failed: %s", hipGetErrorString(e));
         return c.init_status;
     }
-    for (int i = 0; i < kAuxStreams; ++i) {
-        if ((e = hipStreamCreateWithFlags(&c.aux[i], hipStreamNonBlocking)) != hipSuccess ||
+    for (int i = 0; i <= kAuxStreams; ++i) {
+        if ((i < kAuxStreams && (e = hipStreamCreateWithFlags(&c.aux[i], hipStreamNonBlocking)) != hipSuccess) ||
             (e = hipEventCreateWithFlags(&c.join[i], hipEventDisableTiming)) != hipSuccess) {
             c.init_status = fail(OPE_ERR_NO_DEVICE, "HIP initialisation failed: %s", hipGetErrorString(e));
             return c.init_status;
@@ -392,13 +397,13 @@ int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
     const double* d_str = reinterpret_cast<const double*>(dv + o_str);
     const uint2* d_jobs = reinterpret_cast<const uint2*>(dv + o_jobs);
     HIP_TRY(c.slots.ensure(p.n_slots * sizeof(double2)));
-    HIP_TRY(c.out.ensure(b_out));
     HIP_TRY(hipEventRecord(c.ev0, s));
-    // group i runs on stream i mod (1 + kAuxStreams); the auxiliary streams
-    // start after the uploads (ev0) and the reduction waits for all of them.
-    // Every tile writes its own slots, so the overlap changes no result.
-    const int n_aux = std::min<int>(kAuxStreams, (int)p.groups.size() - 1);
-    for (int i = 0; i < n_aux; ++i) HIP_TRY(hipStreamWaitEvent(c.aux[i], c.ev0, 0));
+    // group i runs on lane i mod (1 + kAuxStreams) (lane 0 = the stream, the
+    // others auxiliary streams that start after the uploads, ev0).  Every
+    // tile writes its own slots, so the overlap changes no result.
+    const int n_lanes = std::min<int>(1 + kAuxStreams, (int)p.groups.size());
+    auto lane_stream = [&](int lane) { return lane == 0 ? s : c.aux[lane - 1]; };
+    for (int i = 1; i < n_lanes; ++i) HIP_TRY(hipStreamWaitEvent(c.aux[i - 1], c.ev0, 0));
     for (size_t gi = 0; gi < p.groups.size(); ++gi) {
         const Plan::Group& g = p.groups[gi];
         const size_t lane = gi % (size_t)(1 + kAuxStreams);
@@ -413,15 +418,22 @@ int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
         L.code = g.code;
         HIP_TRY(launch_tiles(L, lane == 0 ? s : c.aux[lane - 1]));
     }
-    for (int i = 0; i < n_aux; ++i) {
-        HIP_TRY(hipEventRecord(c.join[i], c.aux[i]));
-        HIP_TRY(hipStreamWaitEvent(s, c.join[i], 0));
+    // The reduction runs on the lane of the last group, which usually ends
+    // last (each group starts as the previous one's last tiles free CUs), so
+    // that its wait for the other lanes is already met when it reaches the
+    // head of its queue.  Synchronising that lane covers every lane.
+    const int last_lane = p.groups.empty() ? 0 : (int)((p.groups.size() - 1) % (size_t)(1 + kAuxStreams));
+    hipStream_t fin = lane_stream(last_lane);
+    for (int i = 0; i < n_lanes; ++i) {
+        if (i == last_lane) continue;
+        HIP_TRY(hipEventRecord(c.join[i], lane_stream(i)));
+        HIP_TRY(hipStreamWaitEvent(fin, c.join[i], 0));
     }
-    HIP_TRY(hipEventRecord(c.evk, s));
-    HIP_TRY(launch_reduce(d_desc, d_jobs, (uint32_t)p.jobs.size(), c.slots.as<double2>(), c.out.as<double2>(), s));
-    HIP_TRY(hipEventRecord(c.ev1, s));
-    HIP_TRY(hipMemcpyAsync(c.h_out.p, c.out.p, b_out, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipEventRecord(c.evk, fin));
+    HIP_TRY(launch_reduce(d_desc, d_jobs, (uint32_t)p.jobs.size(), c.slots.as<double2>(),
+                          static_cast<double2*>(c.h_out.p), fin));
+    HIP_TRY(hipEventRecord(c.ev1, fin));
+    HIP_TRY(hipStreamSynchronize(fin));
     r.sums.resize(p.n_out);
     std::memcpy(r.sums.data(), c.h_out.p, b_out);
     float ms = 0.0f, kms = 0.0f;
