@@ -403,10 +403,12 @@ int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
     // tile writes its own slots, so the overlap changes no result.
     const int n_lanes = std::min<int>(1 + kAuxStreams, (int)p.groups.size());
     auto lane_stream = [&](int lane) { return lane == 0 ? s : c.aux[lane - 1]; };
-    for (int i = 1; i < n_lanes; ++i) HIP_TRY(hipStreamWaitEvent(c.aux[i - 1], c.ev0, 0));
     for (size_t gi = 0; gi < p.groups.size(); ++gi) {
         const Plan::Group& g = p.groups[gi];
         const size_t lane = gi % (size_t)(1 + kAuxStreams);
+        // an auxiliary lane's wait for the uploads is issued just before its
+        // first launch, so the first group's launch is not queued behind it
+        if (lane > 0 && gi < (size_t)(1 + kAuxStreams)) HIP_TRY(hipStreamWaitEvent(c.aux[lane - 1], c.ev0, 0));
         TileLaunch L;
         L.descs = d_desc;
         L.tiles = d_tiles + g.first;
@@ -416,7 +418,7 @@ int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
         L.fp32 = fp32;
         L.fixed = g.code < kNumCodes ? 1 : 0;
         L.code = g.code;
-        HIP_TRY(launch_tiles(L, lane == 0 ? s : c.aux[lane - 1]));
+        HIP_TRY(launch_tiles(L, lane_stream((int)lane)));
     }
     // The reduction runs on the lane of the last group, which usually ends
     // last (each group starts as the previous one's last tiles free CUs), so
