@@ -423,17 +423,21 @@ __device__ __forceinline__ float ncdf_neg_abs_f32(float z) {
 }
 
 // QE variance step in FP32 (Andersen 2008, psi_c = 1.5) for one sibling, in
-// terms of the conditional mean m, mm2 = 2 m^2 and the conditional variance
-// s2, with no division by psi = s2/m^2; psi > 1.5 <=> s2 > 0.75 mm2.  The two
-// branches share their reciprocals and the final product:
-// * quadratic: t = 2/psi = mm2/s2 (capped at 2e12, the psi floor 1e-12),
-//   b^2 = t - 1 + sqrt(t (t - 1)), v' = m/(1 + b^2) (b + z)^2;
-// * exponential: t = 1 - p = mm2/(s2 + m^2), 1/beta = m/(1 - p), and with
+// terms of the conditional mean m and half the conditional variance, s2h =
+// s2 / 2 (the host halves its coefficients), with no division by psi =
+// s2/m^2 = 2 s2h/m^2; psi > 1.5 <=> s2h > 0.75 m^2.  The two branches share
+// their reciprocals and the final product:
+// * quadratic: t = 2/psi = m^2/s2h (capped at 2e12, the psi floor 1e-12),
+//   1 + b^2 = t + sqrt(t (t - 1)), v' = m/(1 + b^2) (b + z)^2;
+// * exponential: t = 1 - p = m^2/(s2h + m^2/2), 1/beta = m/(1 - p), and with
 //   phic = 1 - ncdf(z): v' = 0 when u <= p (phic >= t), else
 //   ln(t/phic)/beta = m/t max(ln(t/phic), 0), which avoids the cancellation
 //   of 1 - u.
 // qe_sib_f32 returns m r and (b + z)^2 with r = 1/(1 + b^2) or 1/t; the
 // exponential lanes' (b + z)^2 is meaningless and is replaced by the caller.
+// With m = 0 (theta = 0 and v = 0) both m^2 and s2h are 0: t = 0 * inf is
+// NaN, which the cap turns into 2e12 (fminf returns its other operand), and
+// v' = m r (b + z)^2 = 0, as in the reference.
 struct QeSib {
     float mr, bz2, t;
     bool ex;
@@ -441,7 +445,6 @@ struct QeSib {
 
 __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z, float m0, float s0) {
     QeSib o;
-    // s2h = s2 / 2 (the host halves s1 and s0): t = m^2 / s2h or m^2 / (s2h + m^2 / 2)
     const float m = fmaf(d.qe_m1, v, m0), s2h = fmaf(d.qe_s1, v, s0);
     const float msq = m * m;
     o.ex = s2h > 0.75f * msq;
