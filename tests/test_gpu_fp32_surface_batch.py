@@ -23,7 +23,7 @@ def both(ope, payoff, cfg, jumps=None, hp=BASE, expiry=1.0):
 
 
 @pytest.mark.parametrize("payoff", ["asian", "uao", "euro"])
-def test_fp32_within_tenth_se_cfg2(ope, payoff):
+def test_fp32_within_tenth_se_qe_1m_x_365(ope, payoff):
     # Heston QE, 1M paths x 365 steps (the Asian / up-and-out configuration)
     cfg = ope.McConfig(n_paths=1_000_000, n_steps=365, seed=42, scheme=1)
     p = {"asian": ope.Payoff.AsianArithmetic(100.0), "uao": ope.Payoff.UpAndOut(100.0, 130.0),

@@ -69,7 +69,7 @@ def test_qe_reduces_bias_in_feller_violating_regime(ope, ora, precision):
 
 STAT_CASES = [
     # (name, scheme, payoff args, jumps, n_paths, n_steps, rate, q)
-    ("cfg1_heston_euler_euro", 0, ("E", 100.0), None, 100_000, 252, 0.03, 0.0),
+    ("heston_euler_euro_100k_x_252", 0, ("E", 100.0), None, 100_000, 252, 0.03, 0.0),
     ("qe_asian", 1, ("A", 100.0), None, 100_000, 365, 0.03, 0.0),
     ("qe_uao", 1, ("U", 100.0, 130.0), None, 100_000, 365, 0.03, 0.0),
     ("bates_euler_euro_put", 0, ("P", 95.0), (0.5, -0.1, 0.15), 100_000, 252, 0.03, 0.01),

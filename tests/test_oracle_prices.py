@@ -180,7 +180,7 @@ def test_odd_path_count_drops_one_path(ora):
     assert r[4] == 1001
 
 
-def test_baseline_cfg1_against_analytic(ora):
+def test_heston_euler_100k_x_252_against_analytic(ora):
     # Heston European, Euler, FP64, 100k paths, 252 steps, seed 42 (the MC workloads' parameters)
     hp = ora.heston(0.04, 1.5, 0.04, 0.5, -0.7)
     price, se, _, _, count = ora.mc_price(100, 1, 0.03, 0, hp, ora.payoff(), 100_000, 252, 42, True, 0,
