@@ -105,3 +105,23 @@ def test_batch_order_does_not_matter(ope):
     a, _, _ = ope.mc_batch(reqs, 1)
     b, _, _ = ope.mc_batch(reqs[::-1], 1)
     assert a == b[::-1]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n_codes", [4, 5, 6])
+def test_batch_groups_on_every_stream_equal_single_calls(ope, precision, n_codes):
+    # n_codes configurations (scheme x payoff) give one specialised launch
+    # each, spread over the engine's streams; with 5 and 6 groups two of them
+    # share a stream and the reduction runs on the last group's stream
+    hp = ope.HestonParams(**BASE)
+    pays = [ope.Payoff.European(100.0), ope.Payoff.AsianArithmetic(95.0), ope.Payoff.UpAndOut(100.0, 125.0)]
+    reqs = []
+    for i in range(n_codes):
+        cfg = ope.McConfig(n_paths=20_000 + 2_000 * i, n_steps=30 + i, seed=300 + i, scheme=i // 3)
+        reqs.append(ope.McRequest(100.0, 1.0, 0.03, 0.0, hp, pays[i % 3], cfg, 0.0, -0.1, 0.15))
+    for order in (reqs, reqs[::-1]):
+        res, stats, ms = ope.mc_batch(order, precision)
+        for q, r in zip(order, res):
+            one = ope.mc_price_ex(q.spot, q.expiry, q.rate, q.div_yield, q.params, q.payoff, q.cfg,
+                                  (q.lambda_, q.mu_j, q.sigma_j), ope.McOptions(precision))
+            assert (r.price, r.std_error) == (one.result.price, one.result.std_error)
