@@ -125,3 +125,27 @@ def test_batch_groups_on_every_stream_equal_single_calls(ope, precision, n_codes
             one = ope.mc_price_ex(q.spot, q.expiry, q.rate, q.div_yield, q.params, q.payoff, q.cfg,
                                   (q.lambda_, q.mu_j, q.sigma_j), ope.McOptions(precision))
             assert (r.price, r.std_error) == (one.result.price, one.result.std_error)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_fp32_tracks_fp64_in_extreme_regimes(ope, scheme):
+    # high vol-of-vol against slow mean reversion (psi far above 1.5: the QE
+    # exponential branch and its point mass dominate), strong jumps, long
+    # maturity, a put; FP32 and FP64 see the same draws
+    cases = [
+        (dict(v0=0.3, kappa=0.5, theta=0.09, sigma=2.0, rho=-0.9), 5.0, 200, (2.0, -0.2, 0.3)),
+        (dict(v0=0.01, kappa=8.0, theta=0.25, sigma=1.5, rho=0.5), 0.25, 90, None),
+        (dict(v0=0.04, kappa=0.2, theta=0.02, sigma=0.05, rho=-0.3), 10.0, 400, (0.1, 0.05, 0.05)),
+    ]
+    pays = [ope.Payoff.European(110.0, ope.OptionType.Put), ope.Payoff.AsianArithmetic(90.0),
+            ope.Payoff.UpAndOut(100.0, 180.0, 2.0)]
+    for hp, expiry, steps, jumps in cases:
+        cfg = ope.McConfig(n_paths=200_000, n_steps=steps, seed=77, scheme=scheme)
+        for pay in pays:
+            f64 = ope.mc_price_ex(100.0, expiry, 0.02, 0.01, ope.HestonParams(**hp), pay, cfg, jumps,
+                                  ope.McOptions(0)).result
+            f32 = ope.mc_price_ex(100.0, expiry, 0.02, 0.01, ope.HestonParams(**hp), pay, cfg, jumps,
+                                  ope.McOptions(1)).result
+            print(f"extreme {hp} T={expiry} {pay}: fp64={f64} fp32={f32}", flush=True)
+            assert math.isfinite(f32.price) and math.isfinite(f32.std_error)
+            assert abs(f32.price - f64.price) <= 0.1 * f64.std_error, (hp, expiry, pay)
