@@ -529,6 +529,14 @@ __device__ __forceinline__ double payoff_value(const McDesc& d, const PathState&
     return d.disc * (e.ko ? d.rebate : intr);
 }
 
+// Payoff::eval before discounting, for the strike loop: sgn = +1 (call) or -1
+// (put) and nk = -sgn K, so fma(sgn, x, nk) is x - K or K - x with the one
+// rounding of the subtraction, without evaluating both and selecting.
+__device__ __forceinline__ double payoff_undiscounted(const McDesc& d, const PathState& e, double sgn, double nk) {
+    const double intr = fmax(fma(sgn, e.x, nk), 0.0);
+    return e.ko ? d.rebate : intr;
+}
+
 // Draw sources.  Philox: generated in registers.  Replay: read from arrays laid
 // out [step][pair] (coalesced across the wave).
 struct PhiloxSource {
@@ -841,22 +849,27 @@ __device__ __forceinline__ void price_tile(const McDesc& d, uint32_t tile, const
 
     const int lane = tid & 63, wave = tid >> 6;
     const uint32_t ns = d.n_strikes;
+    // the lane's payoffs are summed undiscounted and the sums discounted once
+    // (disc and disc^2): sum and sum of squares of disc * payoff up to rounding
+    const double sgn = d.opt_put ? -1.0 : 1.0, disc2 = d.disc * d.disc;
     for (uint32_t k = 0; k < ns; ++k) {
-        const double K = strikes[d.strike_off + k];
+        const double nk = -sgn * strikes[d.strike_off + k];
         double s = 0.0, q = 0.0;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             if (valid[j]) {
-                const double pa = payoff_value(d, ea[j], K);
+                const double pa = payoff_undiscounted(d, ea[j], sgn, nk);
                 s += pa;
                 q += pa * pa;
                 if (anti) {
-                    const double pb = payoff_value(d, eb[j], K);
+                    const double pb = payoff_undiscounted(d, eb[j], sgn, nk);
                     s += pb;
                     q += pb * pb;
                 }
             }
         }
+        s *= d.disc;
+        q *= disc2;
         wave_sum2(s, q);
         if (lane == 0) lds[k * 4 + wave] = make_double2(s, q);
     }
