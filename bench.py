@@ -512,7 +512,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path has no multi-GPU mode in this build "
-              "(the design shards (request, pair-range) tiles with one 24-byte-per-strike reduction)",
+              "(the design shards (request, pair-range) tiles with one 16-byte-per-strike reduction)",
               file=sys.stderr)
         sys.exit(2)
 
