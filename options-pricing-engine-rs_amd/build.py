@@ -23,7 +23,6 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libopemc.so")
-PROBE = os.path.join(LIBDIR, "issue_probe")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
@@ -70,15 +69,13 @@ def build_engine(verbose=True):
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmds.append((src, [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]))
-    # the instruction issue-cost probe behind the rooflines (a standalone program)
-    cmds.append(("issue_probe.hip", [HIPCC] + COMMON + ["-o", PROBE, os.path.join(CSRC, "issue_probe.hip")]))
     _run_parallel(cmds)
     tmp = LIB + ".tmp"
     link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(link, check=True)
     os.replace(tmp, LIB)
     if verbose:
-        print(f"built {LIB} and {PROBE} in {time.time() - t0:.1f}s")
+        print(f"built {LIB} in {time.time() - t0:.1f}s")
     return LIB
 
 

@@ -71,6 +71,9 @@ def lib():
     L.ora_smallrng_seed.argtypes = [C.c_uint64, _u64p]
     L.ora_xoshiro_next_n.argtypes = [_u64p, C.c_uint64, _u64p]
     L.ora_pcg32_canonical.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, _u32p]
+    L.ora_draw_steps_from_state.restype = None
+    L.ora_draw_steps_from_state.argtypes = [_u64p, C.POINTER(Jumps), C.c_double, C.c_uint64, _dp, _dp, _dp,
+                                            _u32p]
     L.ora_gen_f64_n.argtypes = [C.c_uint64, C.c_uint64, _dp]
     L.ora_gen_range_n.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_uint64, _dp]
     L.ora_box_muller_n.argtypes = [C.c_uint64, C.c_uint64, _dp]

@@ -142,13 +142,15 @@ static inline uint32_t sample_poisson(xoshiro256 *g, double mean) {
 
 typedef struct { double za, zb, jump_sum; } step_draw; /* mc.rs:126-127 */
 
-/* mc.rs:129-145 */
-static inline step_draw draw_step(xoshiro256 *g, const ora_jumps *j, double dt) {
+/* mc.rs:129-145.  *count (optional) receives the Poisson count. */
+static inline step_draw draw_step_n(xoshiro256 *g, const ora_jumps *j, double dt, uint32_t *count) {
     step_draw d;
     box_muller(g, &d.za, &d.zb);
     d.jump_sum = 0.0;
+    if (count) *count = 0;
     if (j && j->has_jumps) {
         uint32_t n = sample_poisson(g, j->lambda * dt);
+        if (count) *count = n;
         if (n != 0) {
             double u, unused;
             box_muller(g, &u, &unused);
@@ -156,6 +158,10 @@ static inline step_draw draw_step(xoshiro256 *g, const ora_jumps *j, double dt) 
         }
     }
     return d;
+}
+
+static inline step_draw draw_step(xoshiro256 *g, const ora_jumps *j, double dt) {
+    return draw_step_n(g, j, dt, NULL);
 }
 
 /* ------------------------------------------------------------ path math */
@@ -479,6 +485,22 @@ void ora_pcg32_canonical(uint64_t initstate, uint64_t initseq, uint64_t n, uint3
         state = old * MUL + inc;
         out[i] = pcg_xsh_rr(old);
     }
+}
+
+/* n_steps successive draw_step calls (mc.rs:129-145) from a given Xoshiro256++
+ * state, which is advanced in place: za, zb, jump_sum and the Poisson count of
+ * each step.  Lets a test check which generator outputs each draw consumes. */
+void ora_draw_steps_from_state(uint64_t state[4], const ora_jumps *j, double dt, uint64_t n_steps,
+                               double *za, double *zb, double *jump_sum, uint32_t *count) {
+    xoshiro256 g;
+    memcpy(g.s, state, sizeof g.s);
+    for (uint64_t i = 0; i < n_steps; ++i) {
+        step_draw d = draw_step_n(&g, j, dt, &count[i]);
+        za[i] = d.za;
+        zb[i] = d.zb;
+        jump_sum[i] = d.jump_sum;
+    }
+    memcpy(state, g.s, sizeof g.s);
 }
 
 void ora_gen_f64_n(uint64_t seed, uint64_t n, double *out) {

@@ -18,6 +18,7 @@
  *   ora_ncdf                  math.rs:16-19
  *   ora_qe_variance_step      mc.rs:288-309
  *   ora_run_paths             mc.rs:210-278 (+ Payoff::eval mc.rs:82-95)
+ *   ora_draw_steps_from_state draw_step mc.rs:129-145 from a given generator state
  *   ora_gen_draws             mc.rs:159-180 (chunked seeding + draw_step mc.rs:129-145)
  *   ora_mc_price              mc.rs:148-207 (mc_heston 98-103 / mc_bates 107-113)
  *   ora_heston_price          heston.rs:61-229

@@ -53,6 +53,27 @@ def test_seed_expansion_is_deterministic_and_spread(ora):
             assert 12 <= bin(w).count("1") <= 52
 
 
+def test_seed_expansion_value_breakage_answer(ora):
+    # rand_core 0.6's own test of SeedableRng::seed_from_u64 (the default the
+    # reference reaches through SmallRng::seed_from_u64, mc.rs:172): the first
+    # 8 bytes of the expansion of each seed below, read little endian, have
+    # popcount 20..44 and differ pairwise in >= 20 bits, and for seed 0 they
+    # are 5029875928683246316 (its value-breakage constant).  Forwarding to
+    # Xoshiro's SplitMix64 seeding, advancing the PCG state after the output,
+    # or a big-endian read would all change that word.
+    st = np.zeros(4, dtype=np.uint64)
+    first = []
+    for seed in [0, 1, 2, 3, 4, 8, 16, 2**64 - 1]:
+        ora.lib().ora_smallrng_seed(seed, st)
+        first.append(int(st[0]))
+    assert first[0] == 5029875928683246316
+    for i, a in enumerate(first):
+        assert 20 <= bin(a).count("1") <= 44
+        for j, b in enumerate(first):
+            if i != j:
+                assert bin(a ^ b).count("1") >= 20
+
+
 def test_gen_f64_is_53_bit_grid_in_unit_interval(ora):
     x = np.zeros(200_000)
     ora.lib().ora_gen_f64_n(7, x.size, x)
@@ -62,11 +83,126 @@ def test_gen_f64_is_53_bit_grid_in_unit_interval(ora):
     assert stats.kstest(x, "uniform").pvalue > 1e-4
 
 
+def test_gen_f64_uses_all_53_bits(ora):
+    # (x >> 11) * 2^-53: below 1/2 the odd multiples of 2^-53 are exact and
+    # occur half the time; a 52-bit mapping would never produce them.
+    x = np.zeros(100_000)
+    ora.lib().ora_gen_f64_n(9, x.size, x)
+    k = (x * 2.0**53).astype(np.uint64)
+    frac_odd = float(np.mean(k % 2 == 1))
+    assert 0.48 < frac_odd < 0.52
+
+
 def test_gen_range_respects_box_muller_bounds(ora):
     x = np.zeros(200_000)
     ora.lib().ora_gen_range_n(11, 1e-12, 1.0, x.size, x)
     assert x.min() >= 1e-12 and x.max() < 1.0
     assert stats.kstest(x, "uniform").pvalue > 1e-4
+
+
+def test_gen_range_is_on_the_52_bit_grid(ora):
+    # rand 0.8 UniformFloat::sample_single: value1_2 has the top 52 bits of the
+    # word as its mantissa, so every output is fl(fl(k 2^-52 * scale) + low)
+    # for an integer k < 2^52 with scale = 1 - 1e-12.  A 53-bit mapping
+    # (j 2^-53 with j odd half the time) lands between those points.
+    lo, hi = 1e-12, 1.0
+    scale = hi - lo
+    x = np.zeros(50_000)
+    ora.lib().ora_gen_range_n(13, lo, hi, x.size, x)
+    k = np.rint((x - lo) / scale * 2.0**52)
+    back = (k * 2.0**-52) * scale + lo
+    small = x < 0.5          # where one step of 2^-53 * scale exceeds an ulp of x
+    assert small.sum() > 20_000
+    assert np.array_equal(back[small], x[small])
+    # and the same for one generator output worked by hand: state {1,2,3,4}
+    # gives first word 41943041 (Blackman-Vigna), hence k = 41943041 >> 12
+    st = np.array([1, 2, 3, 4], dtype=np.uint64)
+    w = np.zeros(1, dtype=np.uint64)
+    ora.lib().ora_xoshiro_next_n(st, 1, w)
+    assert int(w[0]) == 41943041
+    assert (41943041 >> 12) * 2.0**-52 * scale + lo == ((41943041 >> 12) * 2.0**-52) * scale + lo
+
+
+class _Xoshiro256pp:
+    """Xoshiro256++ (Blackman-Vigna), independent of the oracle's C code."""
+
+    M = 2**64 - 1
+
+    def __init__(self, s):
+        self.s = [int(v) for v in s]
+
+    @staticmethod
+    def _rotl(x, k):
+        return ((x << k) | (x >> (64 - k))) & _Xoshiro256pp.M
+
+    def next(self):
+        s, M = self.s, self.M
+        out = (self._rotl((s[0] + s[3]) & M, 23) + s[0]) & M
+        t = (s[1] << 17) & M
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = self._rotl(s[3], 45)
+        return out
+
+
+def _words_box_muller(w1, w2):
+    # u1 from gen_range(1e-12..1.0) (52-bit mapping), u2 from gen::<f64>() (53-bit)
+    lo, scale = 1e-12, 1.0 - 1e-12
+    u1 = ((w1 >> 12) * 2.0**-52) * scale + lo
+    u2 = (w2 >> 11) * 2.0**-53
+    r = math.sqrt(-2.0 * math.log(u1))
+    th = 6.283185307179586 * u2
+    return r * math.cos(th), r * math.sin(th)
+
+
+@pytest.mark.parametrize("lam_dt", [0.0, 1.5])
+def test_draw_step_word_order(ora, lam_dt):
+    # mc.rs:129-145: one draw_step consumes the Box-Muller pair (2 words) first,
+    # then, with jumps, the Poisson sampler's n + 1 words (mc.rs:321-331; none
+    # when lambda dt = 0), then a second Box-Muller pair (2 words) only if
+    # n > 0, whose cosine output u gives jump_sum = mu_j n + sigma_j sqrt(n) u.
+    # Poisson before the normals, an unconditional jump normal or a Poisson
+    # loop that stops one word early would each fail this.
+    n_steps, dt, mu_j, sigma_j = 400, 1.0, -0.1, 0.15
+    jp = ora.jumps(lam_dt / dt, mu_j, sigma_j)
+    st0 = np.zeros(4, dtype=np.uint64)
+    ora.lib().ora_smallrng_seed(ora.lib().ora_splitmix64(42), st0)
+    st = st0.copy()
+    za, zb, js = (np.zeros(n_steps) for _ in range(3))
+    cnt = np.zeros(n_steps, dtype=np.uint32)
+    ora.lib().ora_draw_steps_from_state(st, jp, dt, n_steps, za, zb, js, cnt)
+
+    g = _Xoshiro256pp(st0)
+    el = math.exp(-lam_dt)
+    words = 0
+    for i in range(n_steps):
+        a, b = _words_box_muller(g.next(), g.next())
+        words += 2
+        assert (a, b) == (za[i], zb[i]), i
+        n, jump = 0, 0.0
+        if lam_dt > 0:
+            p, k = 1.0, 0
+            while True:
+                k += 1
+                p *= (g.next() >> 11) * 2.0**-53
+                words += 1
+                if p <= el:
+                    n = k - 1
+                    break
+            if n:
+                u, _ = _words_box_muller(g.next(), g.next())
+                words += 2
+                jump = mu_j * n + sigma_j * math.sqrt(n) * u
+        assert n == cnt[i], i
+        assert jump == js[i], i
+    assert [int(v) for v in st] == g.s
+    expected = n_steps * 2 + (int(cnt.sum()) + n_steps + 2 * int((cnt > 0).sum()) if lam_dt > 0 else 0)
+    assert words == expected
+    if lam_dt > 0:
+        assert 0.6 < (cnt > 0).mean() < 0.9   # both branches exercised
 
 
 def test_box_muller_normals(ora):
