@@ -137,6 +137,34 @@ def draw_uneven_requests(ope, n, seed):
     return reqs
 
 
+def poisson_arrivals(rate, seconds, seed):
+    """Arrival times (s) of a Poisson process at `rate` req/s over `seconds`;
+    rate = inf: 2000 * seconds arrivals at t = 0."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    if math.isinf(rate):
+        return np.zeros(int(2000.0 * seconds))
+    gaps = rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16)
+    arrivals = np.cumsum(gaps)
+    return arrivals[arrivals < seconds]
+
+
+def submit_on_schedule(srv, arrivals, reqs, target):
+    """Submits reqs[i] at arrivals[i] seconds after the start (host clock);
+    returns (tickets, start time)."""
+    tickets = []
+    t0 = time.perf_counter()
+    for a, q in zip(arrivals, reqs):
+        while True:
+            dt = t0 + a - time.perf_counter()
+            if dt <= 0:
+                break
+            time.sleep(dt if dt > 2e-4 else 0)
+        tickets.append(srv.submit(q, target))
+    return tickets, t0
+
+
 def serve(ope, precision, rate, seconds, target, seed=5):
     """Poisson arrivals at `rate` req/s for `seconds`; each request runs until its
     standard error <= target * price (cap: its drawn path count).  rate = inf:
@@ -144,26 +172,12 @@ def serve(ope, precision, rate, seconds, target, seed=5):
     saturated throughput."""
     import numpy as np
 
-    rng = np.random.default_rng(seed)
-    if math.isinf(rate):
-        arrivals = np.zeros(int(2000.0 * seconds))
-    else:
-        gaps = rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16)
-        arrivals = np.cumsum(gaps)
-        arrivals = arrivals[arrivals < seconds]
+    arrivals = poisson_arrivals(rate, seconds, seed)
     reqs = draw_uneven_requests(ope, len(arrivals), seed + 1)
     with ope.Server(precision, max_requests=4096, memory_budget_bytes=64 << 30, idle_exit_ms=200.0) as srv:
         warm = srv.submit(reqs[0], target)   # first launch outside the measured window
         srv.wait(warm)
-        tickets = []
-        t0 = time.perf_counter()
-        for a, q in zip(arrivals, reqs):
-            while True:
-                dt = t0 + a - time.perf_counter()
-                if dt <= 0:
-                    break
-                time.sleep(dt if dt > 2e-4 else 0)
-            tickets.append(srv.submit(q, target))
+        tickets, t0 = submit_on_schedule(srv, arrivals, reqs, target)
         got = [srv.wait(t) for t in tickets]
         wall = time.perf_counter() - t0
         st = srv.stats()

@@ -39,6 +39,9 @@ extern "C" {
 #define OPE_ERR_OVER_BUDGET (-5)
 #define OPE_ERR_NOT_FOUND (-6)
 #define OPE_ERR_SHUTDOWN (-7)
+#define OPE_ERR_CANCELLED (-8) /* serving: the request was cancelled */
+#define OPE_ERR_DEADLINE (-9)  /* serving: the deadline passed before the request was admitted */
+#define OPE_ERR_TIMEOUT (-10)  /* serving: the watchdog saw no round complete for watchdog_ms */
 
 /* OptionType (types.rs:5-13) */
 #define OPE_CALL 0
@@ -214,8 +217,19 @@ typedef struct ope_server_config {
     uint32_t max_requests;        /* resident request slots (0: 1024) */
     uint64_t memory_budget_bytes; /* admission budget for resident request state (0: 64 GiB) */
     uint32_t initial_round_pairs; /* pairs in a request's first round (0: 65536) */
+    uint32_t ring_items;          /* work-ring capacity in 256-pair tiles (0: 2^22); the tiles of
+                                     every resident request, over all its rounds, must fit */
     double idle_exit_ms;          /* the persistent kernel exits after this long without work (0: 200) */
+    double watchdog_ms;           /* with tiles outstanding and no round completing for this long,
+                                     every outstanding request fails with OPE_ERR_TIMEOUT and the
+                                     server stops admitting (0: 30000; negative: off) */
 } ope_server_config;
+
+/* Why a served request stopped (ope_server_request_info.stop_reason). */
+#define OPE_SERVER_STOP_ALL_PATHS 0 /* ran its n_paths */
+#define OPE_SERVER_STOP_TARGET 1    /* std_error <= target_rel_se * |price| */
+#define OPE_SERVER_STOP_DEADLINE 2  /* its deadline passed at a round boundary: the result is the
+                                       estimate from the paths run so far */
 
 typedef struct ope_server_request_info {
     uint64_t paths;        /* paths simulated (both antithetic siblings count) */
@@ -224,6 +238,8 @@ typedef struct ope_server_request_info {
     uint32_t slot;
     double queue_ms;       /* submit -> admission */
     double latency_ms;     /* submit -> result */
+    int32_t stop_reason;   /* OPE_SERVER_STOP_* */
+    uint32_t pad_;
 } ope_server_request_info;
 
 typedef struct ope_server_stats {
@@ -240,6 +256,19 @@ typedef struct ope_server_stats {
 int ope_server_create(const ope_server_config *cfg, ope_server **out);
 /* Queues one request (validated here).  target_rel_se <= 0: run all n_paths. */
 int ope_server_submit(ope_server *s, const ope_mc_request *req, double target_rel_se, uint64_t *ticket);
+/* As ope_server_submit, with a deadline deadline_ms after submission (<= 0:
+ * none).  A request still queued at its deadline fails with OPE_ERR_DEADLINE;
+ * an admitted request stops at the first round boundary past its deadline and
+ * returns the estimate from the paths it ran (stop_reason
+ * OPE_SERVER_STOP_DEADLINE; a request without a target runs in one round, so
+ * the deadline bounds only its wait for admission). */
+int ope_server_submit_ex(ope_server *s, const ope_mc_request *req, double target_rel_se, double deadline_ms,
+                         uint64_t *ticket);
+/* Cancels a request: a queued one fails at once, an admitted one at the end of
+ * the round it is running (its posted tiles still run).  Waiting on it then
+ * returns OPE_ERR_CANCELLED.  OPE_ERR_NOT_FOUND if the ticket is unknown or
+ * has already finished. */
+int ope_server_cancel(ope_server *s, uint64_t ticket);
 /* *done = 1 and the outputs filled once the request has finished. */
 int ope_server_poll(ope_server *s, uint64_t ticket, ope_mc_result *out, ope_server_request_info *info,
                     int32_t *done);

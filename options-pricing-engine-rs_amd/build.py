@@ -2,6 +2,8 @@
 
     python options-pricing-engine-rs_amd/build.py            # engine + oracle
     python options-pricing-engine-rs_amd/build.py --engine   # engine only
+    python options-pricing-engine-rs_amd/build.py --engine --incremental
+                                                  # recompile only stale units
 
 Every HIP translation unit is compiled for gfx950 with packed FP32 math
 disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).  Floating-point
@@ -60,14 +62,26 @@ def _run_parallel(cmds):
         raise RuntimeError(f"build failed: {', '.join(failed)}")
 
 
-def build_engine(verbose=True):
+def _newest_input_mtime():
+    paths = [os.path.join(CSRC, f) for f in os.listdir(CSRC)]
+    inc = os.path.join(ROOT, "include")
+    paths += [os.path.join(inc, f) for f in os.listdir(inc)]
+    return max(os.path.getmtime(p) for p in paths + [__file__])
+
+
+def build_engine(verbose=True, incremental=False):
+    """Compiles every unit (incremental=True: only units whose object is older
+    than some source, header or this script) and links libopemc.so."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     t0 = time.time()
+    newest = _newest_input_mtime() if incremental else None
     cmds, objs = [], []
     for src, extra in UNITS:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > newest:
+            continue
         cmds.append((src, [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]))
     _run_parallel(cmds)
     tmp = LIB + ".tmp"
@@ -88,6 +102,6 @@ def build_oracle(verbose=True):
 if __name__ == "__main__":
     args = set(sys.argv[1:])
     if "--oracle" not in args:
-        build_engine()
+        build_engine(incremental="--incremental" in args)
     if "--engine" not in args:
         build_oracle()

@@ -726,6 +726,9 @@ const char* ope_status_string(int status) {
     case OPE_ERR_OVER_BUDGET: return "over device-memory budget";
     case OPE_ERR_NOT_FOUND: return "unknown ticket";
     case OPE_ERR_SHUTDOWN: return "server shut down";
+    case OPE_ERR_CANCELLED: return "request cancelled";
+    case OPE_ERR_DEADLINE: return "deadline passed before admission";
+    case OPE_ERR_TIMEOUT: return "serving watchdog timeout";
     default: return "unknown status";
     }
 }

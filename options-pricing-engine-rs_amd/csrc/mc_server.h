@@ -74,11 +74,12 @@ struct SrvLaunch {
 
 static_assert(sizeof(SrvItem) == 16, "work item layout");
 
-// mc_server.hip
+// mc_server.hip.  launch_server first resets the launch words of the mirror
+// (zero-initialised device memory of server_mirror_bytes() bytes) on the same
+// stream, then launches the persistent kernel.
 hipError_t launch_server(const SrvLaunch& L, void* mirror, hipStream_t s);
 hipError_t server_occupancy(int fp32, int* blocks_per_cu);
 size_t server_mirror_bytes();
-size_t server_mirror_launch_offset();  // bytes [offset, server_mirror_bytes()) are cleared before a launch
 static_assert(sizeof(SrvRecord) % 4 == 0, "record copied as 32-bit words");
 
 }  // namespace ope

@@ -7,8 +7,14 @@
 //   tail into device memory, so that the other workgroups poll L2 instead of
 //   the host bus.  It alone decides that the kernel drains: on host shutdown,
 //   or when neither the tail nor the ticket counter has moved for idle_limit
-//   polls.  Before it sets the drain flag it records the last tail it
-//   mirrored as the next launch's base.
+//   polls and every item below the mirrored tail has been claimed (so an
+//   idle exit never leaves an item below the tail unclaimed, even when every
+//   resident worker is busy on a long tile).  Before it sets the drain flag
+//   it records the last tail it mirrored as the next launch's base.
+// * Each launch numbers its tickets from launch_base, which k_server_prepare
+//   copies from the previous launch's next_base before the launch (stream
+//   order); the sentinel never writes launch_base, so a workgroup dispatched
+//   late, after the sentinel drained, still numbers its ticket correctly.
 // * A free worker takes the next ticket with one fetch-and-add (tickets are
 //   numbered from that base) and waits until the mirrored tail passes it: the
 //   ticket is the ring index of its next work item.  Idle workers hold the
@@ -35,13 +41,14 @@ namespace {
 constexpr int kRecWords = (int)(sizeof(SrvRecord) / 4);
 
 // Device-memory words shared by the sentinel and the workers.
-// The host clears [tickets, end) before every launch; tail and next_base
-// carry over from the previous launch.
+// k_server_prepare sets launch_base = next_base and clears tickets and
+// shutdown before every launch; tail and next_base carry over.
 struct SrvMirror {
-    uint64_t tail;       // host tail as last seen by the sentinel
-    uint64_t next_base;  // tail at which the last launch drained: tickets start there
-    uint64_t tickets;    // tickets taken in this launch
-    uint32_t shutdown;   // draining
+    uint64_t tail;         // host tail as last seen by the sentinel
+    uint64_t next_base;    // tail at which the last launch drained (written by its sentinel)
+    uint64_t launch_base;  // this launch's first ticket (read-only during the launch)
+    uint64_t tickets;      // tickets taken in this launch
+    uint32_t shutdown;     // draining
     uint32_t pad_;
 };
 
@@ -54,7 +61,8 @@ __device__ __forceinline__ uint32_t sys_load32(const uint32_t* p) {
 
 __device__ void sentinel(const SrvLaunch& L, SrvMirror* mirror) {
     if (threadIdx.x != 0) return;
-    uint64_t last = ~0ull, last_tk = 0, mirrored = mirror->next_base;
+    const uint64_t base = mirror->launch_base;
+    uint64_t last = ~0ull, last_tk = 0, mirrored = base;
     uint32_t idle = 0;
     for (;;) {
         const uint64_t t = sys_load64(&L.hc->tail);
@@ -67,10 +75,13 @@ __device__ void sentinel(const SrvLaunch& L, SrvMirror* mirror) {
             idle = 0;
         } else if (tk != last_tk) {
             idle = 0;
-        } else if (++idle > L.idle_limit) {
-            break;
+        } else if (idle <= L.idle_limit) {
+            ++idle;
+        } else if (base + tk >= mirrored) {
+            break;  // idle, and every item below the mirrored tail has a worker
         }
         last_tk = tk;
+        // the host stops only once every published item has completed
         if (stop) break;
         __builtin_amdgcn_s_sleep(16);
     }
@@ -92,8 +103,8 @@ __global__ __launch_bounds__(kBlock) void k_server(SrvLaunch L, SrvMirror* mirro
         sentinel(L, mirror);
         return;
     }
-    // written by the previous launch's sentinel, which has finished
-    const uint64_t base = __hip_atomic_load(&mirror->next_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // set by k_server_prepare, which ran before this launch
+    const uint64_t base = __hip_atomic_load(&mirror->launch_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
         if (threadIdx.x == 0) {
             const uint64_t ticket =
@@ -169,6 +180,14 @@ __global__ __launch_bounds__(kBlock) void k_server(SrvLaunch L, SrvMirror* mirro
     }
 }
 
+// One thread: this launch's ticket base is where the previous launch drained;
+// its ticket counter and drain flag start at zero.
+__global__ void k_server_prepare(SrvMirror* m) {
+    m->launch_base = m->next_base;
+    m->tickets = 0;
+    m->shutdown = 0;
+}
+
 }  // namespace
 
 hipError_t server_occupancy(int fp32, int* blocks_per_cu) {
@@ -177,11 +196,11 @@ hipError_t server_occupancy(int fp32, int* blocks_per_cu) {
 }
 
 size_t server_mirror_bytes() { return sizeof(SrvMirror); }
-size_t server_mirror_launch_offset() { return offsetof(SrvMirror, tickets); }
 
 hipError_t launch_server(const SrvLaunch& L, void* mirror, hipStream_t s) {
     if (L.n_blocks < 2) return hipErrorInvalidValue;
     SrvMirror* m = static_cast<SrvMirror*>(mirror);
+    hipLaunchKernelGGL(k_server_prepare, dim3(1), dim3(1), 0, s, m);
     if (L.fp32)
         hipLaunchKernelGGL(k_server<float>, dim3(L.n_blocks), dim3(kBlock), 0, s, L, m);
     else

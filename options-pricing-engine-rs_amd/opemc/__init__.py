@@ -34,6 +34,13 @@ OPE_ERR_OUT_OF_MEMORY = -4
 OPE_ERR_OVER_BUDGET = -5
 OPE_ERR_NOT_FOUND = -6
 OPE_ERR_SHUTDOWN = -7
+OPE_ERR_CANCELLED = -8
+OPE_ERR_DEADLINE = -9
+OPE_ERR_TIMEOUT = -10
+
+SERVER_STOP_ALL_PATHS = 0
+SERVER_STOP_TARGET = 1
+SERVER_STOP_DEADLINE = 2
 
 PRECISION_FP64 = 0
 PRECISION_FP32_PATH = 1
@@ -437,12 +444,14 @@ def debug_draws(seed, pair0, n_pairs, n_steps, expiry=1.0, lambda_=0.0, mu_j=0.0
 
 class _ServerConfig(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_requests", C.c_uint32), ("memory_budget_bytes", C.c_uint64),
-                ("initial_round_pairs", C.c_uint32), ("idle_exit_ms", C.c_double)]
+                ("initial_round_pairs", C.c_uint32), ("ring_items", C.c_uint32), ("idle_exit_ms", C.c_double),
+                ("watchdog_ms", C.c_double)]
 
 
 class _ServerRequestInfo(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("path_steps", C.c_uint64), ("rounds", C.c_uint32), ("slot", C.c_uint32),
-                ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+                ("queue_ms", C.c_double), ("latency_ms", C.c_double), ("stop_reason", C.c_int32),
+                ("pad_", C.c_uint32)]
 
 
 class _ServerStats(C.Structure):
@@ -456,14 +465,17 @@ def _server_lib():
     if not getattr(L, "_srv_ready", False):
         L.ope_server_create.argtypes = [C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]
         L.ope_server_submit.argtypes = [C.c_void_p, C.POINTER(_Request), C.c_double, C.POINTER(C.c_uint64)]
+        L.ope_server_submit_ex.argtypes = [C.c_void_p, C.POINTER(_Request), C.c_double, C.c_double,
+                                           C.POINTER(C.c_uint64)]
+        L.ope_server_cancel.argtypes = [C.c_void_p, C.c_uint64]
         L.ope_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(_Result), C.POINTER(_ServerRequestInfo),
                                       C.POINTER(C.c_int32)]
         L.ope_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(_Result),
                                       C.POINTER(_ServerRequestInfo)]
         L.ope_server_get_stats.argtypes = [C.c_void_p, C.POINTER(_ServerStats)]
         L.ope_server_destroy.argtypes = [C.c_void_p]
-        for n in ("ope_server_create", "ope_server_submit", "ope_server_poll", "ope_server_wait",
-                  "ope_server_get_stats", "ope_server_destroy"):
+        for n in ("ope_server_create", "ope_server_submit", "ope_server_submit_ex", "ope_server_cancel",
+                  "ope_server_poll", "ope_server_wait", "ope_server_get_stats", "ope_server_destroy"):
             getattr(L, n).restype = C.c_int
         L._srv_ready = True
     return L
@@ -478,6 +490,7 @@ class ServedResult:
     slot: int
     queue_ms: float
     latency_ms: float
+    stop_reason: int = 0   # SERVER_STOP_*
 
 
 class Server:
@@ -486,24 +499,33 @@ class Server:
     return its result; results do not depend on what else is being served."""
 
     def __init__(self, precision=PRECISION_FP32_PATH, max_requests=1024, memory_budget_bytes=64 << 30,
-                 initial_round_pairs=65536, idle_exit_ms=200.0):
+                 initial_round_pairs=65536, idle_exit_ms=200.0, ring_items=0, watchdog_ms=0.0):
         L = _server_lib()
-        cfg = _ServerConfig(precision, max_requests, memory_budget_bytes, initial_round_pairs, idle_exit_ms)
+        cfg = _ServerConfig(precision, max_requests, memory_budget_bytes, initial_round_pairs, ring_items,
+                            idle_exit_ms, watchdog_ms)
         h = C.c_void_p()
         _check(L.ope_server_create(C.byref(cfg), C.byref(h)))
         self._h = h
         self.precision = precision
 
-    def submit(self, req: McRequest, target_rel_se: float = 0.0) -> int:
+    def submit(self, req: McRequest, target_rel_se: float = 0.0, deadline_ms: float = 0.0) -> int:
+        """Queues a request; deadline_ms > 0 sets a deadline that many
+        milliseconds from now (ope_server_submit_ex)."""
         t = C.c_uint64()
         r = req._c()
-        _check(_server_lib().ope_server_submit(self._h, C.byref(r), target_rel_se, C.byref(t)))
+        if deadline_ms > 0.0:
+            _check(_server_lib().ope_server_submit_ex(self._h, C.byref(r), target_rel_se, deadline_ms, C.byref(t)))
+        else:
+            _check(_server_lib().ope_server_submit(self._h, C.byref(r), target_rel_se, C.byref(t)))
         return t.value
+
+    def cancel(self, ticket: int) -> None:
+        _check(_server_lib().ope_server_cancel(self._h, ticket))
 
     @staticmethod
     def _served(res, info):
         return ServedResult(McResult(res.price, res.std_error), info.paths, info.path_steps, info.rounds, info.slot,
-                            info.queue_ms, info.latency_ms)
+                            info.queue_ms, info.latency_ms, info.stop_reason)
 
     def poll(self, ticket: int):
         res, info, done = _Result(), _ServerRequestInfo(), C.c_int32()

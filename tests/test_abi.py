@@ -61,6 +61,11 @@ def test_struct_layout(ope):
     assert C.sizeof(ope._Result) == 16
     assert C.sizeof(ope._Stats) == 32
     assert C.sizeof(ope._Request) == 4 * 8 + 40 + 3 * 8 + 32 + 32
+    # ope_server_config: ring_items sits in the slot that was padding before idle_exit_ms
+    assert C.sizeof(ope._ServerConfig) == 40
+    assert ope._ServerConfig.ring_items.offset == 20 and ope._ServerConfig.idle_exit_ms.offset == 24
+    assert ope._ServerConfig.watchdog_ms.offset == 32
+    assert C.sizeof(ope._ServerRequestInfo) == 48 and ope._ServerRequestInfo.stop_reason.offset == 40
 
 
 def test_config_default_is_reference_default(ope):
@@ -115,8 +120,9 @@ def test_invalid_arguments_rejected_on_host(ope, case):
 
 def test_status_strings(ope):
     L = ope.lib()
-    for code in range(0, -8, -1):
-        assert L.ope_status_string(code)
+    L.ope_status_string.restype = C.c_char_p
+    for code in range(0, -11, -1):
+        assert L.ope_status_string(code) not in (None, b"unknown status")
     with pytest.raises(ope.OpeError) as e:
         ope.mc_heston(100.0, 1.0, 0.03, 0.0, _hp(ope), ope.Payoff.European(100.0), ope.McConfig(n_paths=1))
     assert e.value.status == ope.OPE_ERR_INVALID_ARGUMENT
