@@ -161,6 +161,21 @@ int ope_mc_bates_ex(double spot, double expiry, double rate, double div_yield,
                     const ope_payoff *payoff, const ope_mc_config *cfg,
                     const ope_mc_options *opts, ope_mc_result *out);
 
+/* No counterpart in the reference.  ope_mc_bates_ex for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the call allocates nothing; its device buffers live in a
+ * fixed 385 KiB array in the library's device code (one per GPU, held until
+ * the process ends).  The first call on a device also initialises the
+ * library's context and looks the array up.  Philox draws only (no replay,
+ * no per-path outputs); at most 2^24 antithetic pairs (2^25 paths), larger
+ * requests fail with OPE_ERR_INVALID_ARGUMENT.  The result equals
+ * ope_mc_bates_ex's bit for bit (lambda == 0: ope_mc_heston's).  Calls are
+ * serialised by the library; the array keeps the last call's contents. */
+int ope_mc_bates_noalloc(double spot, double expiry, double rate, double div_yield,
+                         const ope_heston_params *params, double lambda, double mu_j, double sigma_j,
+                         const ope_payoff *payoff, const ope_mc_config *cfg,
+                         const ope_mc_options *opts, ope_mc_result *out);
+
 /* European strike x expiry surface from shared paths.  One simulation per
  * expiry (dt = expiry / cfg->n_steps, the reference's semantics), every
  * strike priced from the same paths.  out[e * n_strikes + k] equals what

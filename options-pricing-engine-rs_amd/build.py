@@ -34,6 +34,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
 UNITS = [
     ("mc_kernels.hip", ["-ffp-contract=on"]),
     ("mc_server.hip", ["-ffp-contract=on"]),
+    ("mc_noalloc.hip", ["-ffp-contract=on"]),
     ("mc_replay.hip", ["-ffp-contract=off"]),
     ("cf_kernels.hip", ["-ffp-contract=off"]),
     ("bsm_kernels.hip", ["-ffp-contract=off"]),

@@ -298,6 +298,27 @@ struct Plan {
 // launch each; more than that share one mixed launch.
 constexpr size_t kMaxFixedGroups = 6;
 
+// Pairs per thread and tile count of a request (a function of the request
+// alone, so a request computes the same bits in every entry point).
+int set_tiling(McDesc& d) {
+    d.ppt = d.n_pairs >= kPPT4Threshold ? 4 : 1;
+    const uint64_t per_tile = (uint64_t)kBlock * d.ppt;
+    const uint64_t nt = (d.n_pairs + per_tile - 1) / per_tile;
+    if (nt > 0x7fffffffull) return fail(OPE_ERR_INVALID_ARGUMENT, "too many tiles");
+    d.n_tiles = (uint32_t)nt;
+    return OPE_OK;
+}
+
+int set_single_tiling(McDesc& d) {
+    const int rc = set_tiling(d);
+    if (rc) return rc;
+    d.n_strikes = 1;
+    d.strike_off = 0;
+    d.out_off = 0;
+    d.slot_off = 0;
+    return OPE_OK;
+}
+
 int plan(std::vector<Job>& in, Plan& p) {
     p = Plan();
     uint64_t slot = 0;
@@ -309,11 +330,9 @@ int plan(std::vector<Job>& in, Plan& p) {
             return fail(OPE_ERR_INVALID_ARGUMENT, "strike count %zu outside [1, %d]", ns, kMaxStrikes);
         for (double k : in[i].strikes)
             if (!std::isfinite(k)) return fail(OPE_ERR_INVALID_ARGUMENT, "non-finite strike");
-        d.ppt = d.n_pairs >= kPPT4Threshold ? 4 : 1;
-        const uint64_t per_tile = (uint64_t)kBlock * d.ppt;
-        const uint64_t nt = (d.n_pairs + per_tile - 1) / per_tile;
-        if (nt > 0x7fffffffull) return fail(OPE_ERR_INVALID_ARGUMENT, "too many tiles");
-        d.n_tiles = (uint32_t)nt;
+        int rc = set_tiling(d);
+        if (rc) return rc;
+        const uint64_t nt = d.n_tiles;
         d.n_strikes = (uint32_t)ns;
         d.strike_off = (uint32_t)p.strikes.size();
         d.out_off = out;
@@ -468,7 +487,7 @@ void finish(const McDesc& d, double2 sums, ope_mc_result* out, ope_mc_stats* st)
     }
 }
 
-int check_opts(const ope_mc_options* o) {
+int check_options(const ope_mc_options* o) {
     if (!o) return OPE_OK;
     if (o->precision != OPE_PRECISION_FP64 && o->precision != OPE_PRECISION_FP32_PATH)
         return fail(OPE_ERR_INVALID_ARGUMENT, "unknown precision %d", o->precision);
@@ -557,7 +576,7 @@ int price_replay(Context& c, Job& job, const ope_mc_options* o, ope_mc_result* o
 
 int price_single(const Request& r, const ope_mc_options* o, ope_mc_result* out) {
     if (!out) return fail(OPE_ERR_INVALID_ARGUMENT, "null output");
-    int rc = check_opts(o);
+    int rc = check_options(o);
     if (rc) return rc;
     Job job;
     if ((rc = make_desc(r, job.d))) return rc;
@@ -596,6 +615,14 @@ Request make_request(double spot, double expiry, double rate, double div_yield, 
 int device_available() {
     std::lock_guard<std::mutex> lk(g_ctx.mu);
     return ensure_init(g_ctx);
+}
+
+int with_context(int (*body)(void* arg, const ContextView& view), void* arg) {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    const int rc = ensure_init(g_ctx);
+    if (rc) return rc;
+    const ContextView v{g_ctx.stream, g_ctx.ev0, g_ctx.evk, g_ctx.ev1};
+    return body(arg, v);
 }
 
 }  // namespace host
@@ -648,7 +675,7 @@ int ope_mc_surface(double spot, double rate, double div_yield, const ope_heston_
                    ope_mc_result* out) {
     if (!params || !cfg || !expiries || !strikes || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
     if (n_expiries < 1 || n_strikes < 1) return fail(OPE_ERR_INVALID_ARGUMENT, "empty surface");
-    int rc = check_opts(opts);
+    int rc = check_options(opts);
     if (rc) return rc;
     if (opts && opts->rng == OPE_RNG_REPLAY) return fail(OPE_ERR_INVALID_ARGUMENT, "surface does not take replay draws");
     std::vector<Job> jobs(n_expiries);
@@ -677,7 +704,7 @@ int ope_mc_surface(double spot, double rate, double div_yield, const ope_heston_
 int ope_mc_batch(const ope_mc_request* reqs, uint32_t n, const ope_mc_options* opts, ope_mc_result* out) {
     if (!reqs || !out) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
     if (n == 0) return OPE_OK;
-    int rc = check_opts(opts);
+    int rc = check_options(opts);
     if (rc) return rc;
     if (opts && opts->rng == OPE_RNG_REPLAY) return fail(OPE_ERR_INVALID_ARGUMENT, "batch does not take replay draws");
     std::vector<Job> jobs(n);

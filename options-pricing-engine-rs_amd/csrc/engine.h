@@ -38,6 +38,27 @@ Request make_request(double spot, double expiry, double rate, double div_yield, 
 // OPE_OK when a HIP device is usable (initialises the shared context once).
 int device_available();
 
+// The shared context's stream and timing events, as a call sees them.
+struct ContextView {
+    hipStream_t stream;
+    hipEvent_t ev0, evk, ev1;  // before the pricing launch, after it, after the reduction
+};
+
+// Locks the shared context (every entry point is serialised by it),
+// initialises it on first use, and runs body(arg, view) under the lock.
+int with_context(int (*body)(void* arg, const ContextView& view), void* arg);
+
+// Checks an options struct (precision and RNG mode).
+int check_options(const ope_mc_options* o);
+
+// A single request's tiling as every entry point plans it: pairs per thread
+// (4 from 2^20 pairs, else 1), tiles of kBlock * ppt pairs, one strike.
+// Returns OPE_OK or OPE_ERR_INVALID_ARGUMENT for too many tiles.
+int set_single_tiling(McDesc& d);
+
+// Price and standard error from (sum, sum_sq), exactly as mc.rs:198-204.
+void finish(const McDesc& d, double2 sums, ope_mc_result* out, ope_mc_stats* st);
+
 }  // namespace host
 }  // namespace ope
 
