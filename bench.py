@@ -273,7 +273,7 @@ def analytic_cf(ope, wm, steps=5, with_cpu=True):
         out["max_abs_diff_vs_oracle"] = float(np.max(np.abs(ref - prices)))
         out["cpu_baseline"] = {"value": n / cpu_wall, "unit": "options/s", "cores": cores, "kind": "port",
                                "sample": f"the same {n} options, C restatement of heston.rs / bates.rs, "
-                                         f"{cores} threads, {cpu_wall:.2f} s wall"}
+                                         f"{cores} threads, {cpu_wall:.2f} s wall", **host_info()}
     return out
 
 
@@ -337,7 +337,7 @@ def bsm_iv(ope, wm, steps=5, with_cpu=True, n_bsm=1 << 22, n_iv=1 << 20):
         out["max_rel_diff_vs_oracle"] = float(np.max(np.abs(g["price"][:m] - ref[:, 0]) / np.maximum(1.0, ref[:, 0])))
         out["cpu_baseline"] = {"bsm_options_per_s": m / t_bsm, "iv_options_per_s": m / t_iv, "cores": cores,
                                "kind": "port", "sample": f"the first {m} contracts, C restatement of bsm.rs / iv.rs, "
-                                                         f"{cores} threads"}
+                                                         f"{cores} threads", **host_info()}
     return out
 
 
@@ -391,7 +391,7 @@ def calibration_eval(ope, steps=5, with_cpu=True):
             entry["max_rel_diff_vs_oracle"] = float(np.max(np.abs(jac - ref[1]) / np.maximum(1.0, np.abs(ref[1]))))
             entry["cpu_baseline"] = {"ms_per_eval": cpu * 1e3, "cores": 1, "kind": "port",
                                      "sample": "the same evaluation, C restatement of calibration.rs, one thread "
-                                               "(the reference's evaluation loop is sequential)"}
+                                               "(the reference's evaluation loop is sequential)", **host_info()}
         out[name] = entry
         print(f"# calibration {name}: {wall * 1e3:.2f} ms per evaluation ({entry['prices_per_eval']} prices)",
               file=sys.stderr, flush=True)
@@ -440,7 +440,10 @@ def host_info():
                     break
     except OSError:
         pass
-    return {"cpu_model": model, "nproc": os.cpu_count()}
+    import oracle
+
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "build": oracle.lib().ora_build_flags().decode()}
 
 
 def cpu_uneven_sample(ope, cores, budget=6e8):

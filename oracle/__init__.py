@@ -66,6 +66,8 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
     L = C.CDLL(LIB_PATH)
+    L.ora_build_flags.restype = C.c_char_p
+    L.ora_build_flags.argtypes = []
     L.ora_splitmix64.restype = C.c_uint64
     L.ora_splitmix64.argtypes = [C.c_uint64]
     L.ora_smallrng_seed.argtypes = [C.c_uint64, _u64p]

@@ -458,6 +458,12 @@ int ora_run_paths(double spot, double expiry, double rate, double div_yield, con
 
 /* ------------------------------------------------ primitives for tests */
 
+#ifndef ORA_BUILD_FLAGS
+#define ORA_BUILD_FLAGS "unknown"
+#endif
+/* The compiler and flags this library was built with. */
+const char *ora_build_flags(void) { return ORA_BUILD_FLAGS; }
+
 void ora_smallrng_seed(uint64_t seed, uint64_t state_out[4]) {
     xoshiro256 g;
     smallrng_seed_from_u64(&g, seed);

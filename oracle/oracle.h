@@ -32,6 +32,7 @@
  *   ora_calib_jtj_grad        calibration.rs:655-667 (calib.c)
  *   ora_bsm_price             bsm.rs:7-12, 55-60
  *   ora_merton_price          textbook Merton (1976) series (no reference counterpart)
+ *   ora_build_flags           the compiler command line the library was built with
  */
 #ifndef OPE_ORACLE_H
 #define OPE_ORACLE_H
