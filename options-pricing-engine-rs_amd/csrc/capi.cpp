@@ -264,7 +264,7 @@ int make_desc(const Request& r, McDesc& d) {
         d.pois_e0 = std::exp(-pois_mu);
         const double thr = std::ceil(d.pois_e0 * 4294967296.0 - 0.5);
         d.pois_thr0 = thr >= 4294967295.0 ? 0xFFFFFFFFu : (thr <= 0.0 ? 0u : (uint32_t)thr);
-        d.pois_thr0_hi = d.pois_thr0 >> 16;
+        d.pois_gate = d.pois_thr0 >> 23;
         d.mu_j = r.mu_j;
         d.sigma_j = r.sigma_j;
         d.f_mu_j2 = (float)(kLog2eD * r.mu_j);

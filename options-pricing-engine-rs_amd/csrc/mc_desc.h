@@ -36,7 +36,7 @@ struct alignas(16) McDesc {
     uint32_t strike_off;  // into the strike array
     uint32_t out_off;     // into the (sum, sum_sq) result array
     uint32_t pois_thr0;   // w < thr0  <=>  (w + 0.5) 2^-32 < e^-mu  (Poisson count 0)
-    uint32_t pois_thr0_hi;  // thr0 >> 16: a high half below it means count 0 whatever the low half
+    uint32_t pois_gate;   // thr0 >> 23: 9 high bits below it mean count 0 whatever the low bits
     uint32_t pad_;
 
     // ---- FP64 reference-arithmetic constants

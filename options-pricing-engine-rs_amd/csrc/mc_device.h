@@ -31,13 +31,15 @@ constexpr uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
 constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
 
 // Substreams of one pair (the counter's fourth word):
-//   kTagNormal   call s/2: words (x,y) -> normals of step s for even s, (z,w) for odd s
-//   kTagPoisson  call s/8: word (s/2)%4, high half for even s, low half for
-//                odd s -> the 16 high bits of the Poisson uniform of step s (Bates)
-//   kTagJump     call s:   word z -> its 16 low bits, when the high bits
-//                          allow a non-zero count; words (x,y) -> the
+//   kTagNormal   call s/2: words (x,y) -> normals of step s for even s, (z,w)
+//                for odd s; the angle word's 23 high bits give the angle, its
+//                9 low bits the 9 high bits of the Poisson uniform of step s
+//                (Bates)
+//   kTagJump     call s:   word z -> the uniform's 23 low bits, when the high
+//                          bits allow a non-zero count; words (x,y) -> the
 //                          jump-size normal of step s when the count is non-zero
-constexpr uint32_t kTagNormal = 0u, kTagPoisson = 1u, kTagJump = 2u;
+constexpr uint32_t kTagNormal = 0u, kTagJump = 2u;
+constexpr uint32_t kGateBits = 9u, kGateMask = (1u << kGateBits) - 1u;
 
 // a ^ b ^ c in one v_bitop3_b32 (truth table 0x96, the parity of the three
 // inputs).  The instruction takes at most one scalar operand, so it is used
@@ -91,13 +93,15 @@ constexpr double kLog2e = 1.4426950408889634;
 constexpr double kLn2 = 0.6931471805599453;
 
 // --------------------------------------------------------- normal draws
-// Box-Muller on two 32-bit words: u1 = (w0 + 1/2) 2^-32 in (0,1), u2 = w1 2^-32
-// in [0,1); za = r cos(2 pi u2), zb = r sin(2 pi u2), r = sqrt(-2 ln u1).
-// Both precisions consume the same words, so FP32 and FP64 runs see the
-// same normals up to rounding.
+// Box-Muller on two 32-bit words: u1 = (w0 + 1/2) 2^-32 in (0,1), u2 =
+// (w1 >> 9) 2^-23 in [0,1) (a 23-bit angle: the FP32 cosine cannot use more,
+// and the angle word's 9 low bits serve as the Poisson gate of the step);
+// za = r cos(2 pi u2), zb = r sin(2 pi u2), r = sqrt(-2 ln u1).  Both
+// precisions consume the same words, so FP32 and FP64 runs see the same
+// normals up to rounding.
 __device__ __forceinline__ void box_muller_f64(uint32_t w0, uint32_t w1, double& za, double& zb) {
     const double u1 = ((double)w0 + 0.5) * 0x1p-32;
-    const double u2 = (double)w1 * 0x1p-32;
+    const double u2 = (double)(w1 >> kGateBits) * 0x1p-23;
     const double r = sqrt(-2.0 * log(u1));
     double s, c;
     sincospi(2.0 * u2, &s, &c);
@@ -105,12 +109,20 @@ __device__ __forceinline__ void box_muller_f64(uint32_t w0, uint32_t w1, double&
     zb = r * s;
 }
 
+// 1 + u2 as an FP32 value in [1, 2), exactly: the angle bits w1 >> 9 become
+// the mantissa below the exponent of 1.0 in one v_alignbit_b32.  The
+// hardware cosine and sine take revolutions and have period 1, so they take
+// 1 + u2 for u2.
+__device__ __forceinline__ float angle_rev_f32(uint32_t w1) {
+    return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, w1, kGateBits));
+}
+
 __device__ __forceinline__ void box_muller_f32(uint32_t w0, uint32_t w1, float& za, float& zb) {
     const float u1 = fmaf((float)w0, 0x1p-32f, 0x1p-33f);
-    const float u2 = (float)w1 * 0x1p-32f;
+    const float a = angle_rev_f32(w1);
     const float r = hw_sqrt(-1.38629436111989061f * hw_log2(u1));  // -2 ln2 log2(u1)
-    za = r * hw_cos_rev(u2);
-    zb = r * hw_sin_rev(u2);
+    za = r * hw_cos_rev(a);
+    zb = r * hw_sin_rev(a);
 }
 
 // Exact Poisson count by inverse CDF of (w + 1/2) 2^-32.  The common n = 0
@@ -145,10 +157,10 @@ __device__ __forceinline__ PairStream make_stream(const McDesc& d, uint64_t pair
     return s;
 }
 
-// Jump of step s given the 16 high bits h of its Poisson uniform word, for
-// h >= thr0 >> 16 (the caller's test: below it the count is 0 whatever the
+// Jump of step s given the 9 high bits h of its Poisson uniform word, for
+// h >= thr0 >> 23 (the caller's test: below it the count is 0 whatever the
 // low bits): jump_sum = mu_j n + sigma_j sqrt(n) z (mc.rs:139-140).  The
-// jump call of step s supplies the low 16 bits (word 2), completing the
+// jump call of step s supplies the low 23 bits (word 2), completing the
 // 32-bit word W of the exact inverse-CDF count, and the jump normal (the
 // Box-Muller cosine of words 0, 1) when the count is non-zero.  FP64 mode
 // returns the jump in ln units, FP32 mode in log2 units.
@@ -159,7 +171,7 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
     R j = 0;
     n = 0;
     const uint4 w = st.call(s, kTagJump);
-    const uint32_t W = (h << 16) | (w.z & 0xFFFFu);
+    const uint32_t W = (h << (32u - kGateBits)) | (w.z >> kGateBits);
     if (W >= d.pois_thr0) {
         n = poisson_tail(W, d.pois_mu, d.pois_e0);
         if constexpr (FP32) {
@@ -180,18 +192,18 @@ struct NoBlockEnd {
 };
 
 // Drives the Philox-mode draws of one pair through all steps, calling
-// step(za, zb, jump, n, s) once per step s.  Heston: one Philox call per two
-// steps.  Bates: the same normals plus one Poisson call per eight steps (16
-// bits per step) and, only for a step whose count may be non-zero, one jump
-// call.
+// step(za, zb, jump, n, s) once per step s: one Philox call per two steps.
+// Bates: the 9 low bits of a step's angle word are the high bits of its
+// Poisson uniform; below thr0 >> 23 (probability e^{-lambda dt} up to 2^-9)
+// the count is 0, and only otherwise does the step draw its jump call.
 //
 // DEFER (FP32 European payoffs, whose log-price needs only the sum of the
-// jumps): step() gets jump 0, and a step with a non-zero count only records
-// (step, Poisson word) in one of two per-lane slots; the jumps of the slots
-// are drawn after the last step and added to *jsum.  A wave then runs the
-// rare jump code about twice per path instead of at every step where one of
-// its lanes jumps.  A lane's third and later events are drawn at once.  The
-// draws are the same words either way.
+// jumps): step() gets jump 0, and a step whose gate bits allow a count
+// only records (step, gate bits) in one of three per-lane slots; the jumps
+// of the slots are drawn after the last step and added to *jsum.  A wave then
+// runs the rare jump code about three times per path instead of at every
+// step where one of its lanes may jump.  A lane's fourth and later events
+// are drawn at once.  The draws are the same words either way.
 //
 // block_end(s) is called after step s for every s = 8k + 7 of the steps
 // that complete a block of eight (FP32 mode, in which the step sums its
@@ -210,22 +222,18 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     using R = typename std::conditional<FP32, float, double>::type;
     static_assert(!DEFER || (FP32 && JUMPS), "deferred jumps are an FP32 jump-model mode");
     static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
-    uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u;
+    uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u, es2 = ~0u, ew2 = 0u;
     float jd = 0.0f;
     const float phi = MIX ? in_vgpr(d.eu_phi) : 0.0f;
-    // pw: the Poisson word of the step, its high half (lo false) or its low
-    // half (lo true) holding the step's 16 bits; tested without extracting
-    // the half: pw >= thr << 16 for the high half, a 16-bit compare for the
-    // low half
-    const uint32_t thr_hi = d.pois_thr0_hi, thr_hi16 = thr_hi << 16;
-    auto one = [&](uint32_t wa, uint32_t wb, uint32_t pw, bool lo, uint32_t s, bool end = false) {
+    const uint32_t gate = d.pois_gate;
+    auto one = [&](uint32_t wa, uint32_t wb, uint32_t s) {
         R za, zb;
         if constexpr (MIX) {
             const float u1 = fmaf((float)wa, 0x1p-32f, 0x1p-33f);
-            const float wf = (float)wb;
+            const float a = angle_rev_f32(wb);
             const float ra = hw_sqrt(d.eu_r2 * hw_log2(u1));  // a2 sqrt(-2 ln u1)
-            za = ra * hw_cos_rev(wf * 0x1p-32f);
-            zb = (ra * d.eu_kb) * hw_cos_rev(fmaf(wf, 0x1p-32f, -phi));
+            za = ra * hw_cos_rev(a);
+            zb = (ra * d.eu_kb) * hw_cos_rev(a - phi);
         } else if constexpr (FP32) {
             box_muller_f32(wa, wb, za, zb);
         } else {
@@ -234,9 +242,8 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
         uint32_t n = 0;
         R j = 0;
         if constexpr (JUMPS) {
-            const bool maybe = lo ? (uint16_t)pw >= (uint16_t)thr_hi : pw >= thr_hi16;
-            if (maybe) {  // rare: P ~ 1 - e^{-lambda dt} + 2^-16
-                const uint32_t h = lo ? (pw & 0xFFFFu) : (pw >> 16);
+            const uint32_t h = wb & kGateMask;
+            if (h >= gate) {  // rare: P ~ 1 - e^{-lambda dt} + 2^-9
                 if constexpr (DEFER) {
                     if (es0 == ~0u) {
                         es0 = s;
@@ -244,6 +251,9 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
                     } else if (es1 == ~0u) {
                         es1 = s;
                         ew1 = h;
+                    } else if (es2 == ~0u) {
+                        es2 = s;
+                        ew2 = h;
                     } else {
                         uint32_t nn;
                         jd += jump_of_step<true>(d, st, s, h, nn);
@@ -254,77 +264,46 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
             }
         }
         step(za, zb, j, n, s);
-        if (end) block_end(s);
     };
     const uint32_t N = d.n_steps;
     uint32_t i = 0;
-    if constexpr (JUMPS) {
-        // Poisson halves of step s: word (s >> 1) & 3 of call s >> 3, high
-        // half for even s, low half for odd s
-        auto duo = [&](uint32_t pw, uint32_t s, bool end = false) {  // steps s (even) and s + 1
-            const uint4 w = st.call(s >> 1, kTagNormal);
-            one(w.x, w.y, pw, false, s);
-            one(w.z, w.w, pw, true, s + 1, end);
-        };
+    if constexpr (FP32) {
+        // blocks of eight steps; FP32 Euler unrolls a block (the scheduler
+        // overlaps consecutive steps), the QE step, three times longer,
+        // measured slower unrolled and keeps a two-step body
         for (; i + 8 <= N; i += 8) {
-            const uint4 p = st.call(i >> 3, kTagPoisson);
-            duo(p.x, i);
-            duo(p.y, i + 2);
-            duo(p.z, i + 4);
-            duo(p.w, i + 6, true);
-        }
-        if (i < N) {  // 1..7 steps left
-            const uint4 p = st.call(i >> 3, kTagPoisson);
-            const uint32_t pw[4] = {p.x, p.y, p.z, p.w};
+            if constexpr (MIX) {
 #pragma unroll
-            for (uint32_t k = 0; k < 4; ++k) {
-                const uint32_t s = i + 2u * k;
-                if (s + 1 < N) {
-                    duo(pw[k], s);
-                } else if (s < N) {
-                    const uint4 w = st.call(s >> 1, kTagNormal);
-                    one(w.x, w.y, pw[k], false, s);
+                for (uint32_t k = 0; k < 8; k += 2) {
+                    const uint4 w = st.call((i + k) >> 1, kTagNormal);
+                    one(w.x, w.y, i + k);
+                    one(w.z, w.w, i + k + 1);
                 }
-            }
-        }
-    } else {
-        if constexpr (FP32) {
-            // blocks of eight steps; FP32 Euler unrolls a block (the
-            // scheduler overlaps consecutive steps), the QE step, three times
-            // longer, measured slower unrolled and keeps a two-step body
-            for (; i + 8 <= N; i += 8) {
-                if constexpr (MIX) {
-#pragma unroll
-                    for (uint32_t k = 0; k < 8; k += 2) {
-                        const uint4 w = st.call((i + k) >> 1, kTagNormal);
-                        one(w.x, w.y, 0u, false, i + k);
-                        one(w.z, w.w, 0u, true, i + k + 1);
-                    }
-                } else {
+            } else {
 #pragma nounroll
-                    for (uint32_t k = 0; k < 8; k += 2) {
-                        const uint4 w = st.call((i + k) >> 1, kTagNormal);
-                        one(w.x, w.y, 0u, false, i + k);
-                        one(w.z, w.w, 0u, true, i + k + 1);
-                    }
+                for (uint32_t k = 0; k < 8; k += 2) {
+                    const uint4 w = st.call((i + k) >> 1, kTagNormal);
+                    one(w.x, w.y, i + k);
+                    one(w.z, w.w, i + k + 1);
                 }
-                block_end(i + 7);
             }
+            block_end(i + 7);
         }
-        for (; i + 2 <= N; i += 2) {
-            const uint4 w = st.call(i >> 1, kTagNormal);
-            one(w.x, w.y, 0u, false, i);
-            one(w.z, w.w, 0u, true, i + 1);
-        }
-        if (i < N) {
-            const uint4 w = st.call(i >> 1, kTagNormal);
-            one(w.x, w.y, 0u, false, i);
-        }
+    }
+    for (; i + 2 <= N; i += 2) {
+        const uint4 w = st.call(i >> 1, kTagNormal);
+        one(w.x, w.y, i);
+        one(w.z, w.w, i + 1);
+    }
+    if (i < N) {
+        const uint4 w = st.call(i >> 1, kTagNormal);
+        one(w.x, w.y, i);
     }
     if constexpr (DEFER) {
         uint32_t nn;
         if (es0 != ~0u) jd += jump_of_step<true>(d, st, es0, ew0, nn);
         if (es1 != ~0u) jd += jump_of_step<true>(d, st, es1, ew1, nn);
+        if (es2 != ~0u) jd += jump_of_step<true>(d, st, es2, ew2, nn);
         *jsum += jd;
     }
 }
@@ -483,7 +462,10 @@ __device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& v
         const bool pos = za >= 0.0f;
         // z = za for sibling a, z = -za for sibling b; with the sibling, one
         // reciprocal serves both: 1/e = (1 - e) r and 1/(1 - e) = e r for
-        // r = 1/(e (1 - e)), e in (0, 1/2]
+        // r = 1/(e (1 - e)), e in (0, 1/2].  (One logarithm per lane, of the
+        // exponential sibling's argument, with a second pass for lanes where
+        // both siblings are exponential, measured 0.9 more transcendentals
+        // per pair-step: some lane of a wave almost always has both.)
         if (ANTI) {
             const float r = hw_rcp(e * ome);
             const float la = fmaxf(hw_log2(a.t * ((pos ? ome : e) * r)), 0.0f) * 0.69314718055994531f;

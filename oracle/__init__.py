@@ -317,7 +317,8 @@ def ncdf(x):
 M32 = 0xFFFFFFFF
 PHILOX_M0, PHILOX_M1 = 0xD2511F53, 0xCD9E8D57
 PHILOX_W0, PHILOX_W1 = 0x9E3779B9, 0xBB67AE85
-TAG_NORMAL, TAG_POISSON, TAG_JUMP = 0, 1, 2
+TAG_NORMAL, TAG_JUMP = 0, 2
+GATE_BITS = 9
 
 
 def philox4x32_10(c0, c1, c2, c3, k0, k1):
@@ -366,7 +367,7 @@ def poisson_from_word(w, mu, kmax=1024):
 
 def _box_muller(wa, wb):
     u1 = (wa.astype(np.float64) + 0.5) * 2.0 ** -32
-    u2 = wb.astype(np.float64) * 2.0 ** -32
+    u2 = (wb >> np.uint64(GATE_BITS)).astype(np.float64) * 2.0 ** -23
     r = np.sqrt(-2.0 * np.log(u1))
     return r * np.cos(2.0 * np.pi * u2), r * np.sin(2.0 * np.pi * u2)
 
@@ -375,15 +376,15 @@ def philox_draws(seed, pair_ids, n_steps, dt=None, lam=0.0, mu_j=0.0, sigma_j=0.
     """The GPU engine's Philox-mode draws, FP64, for the given pair indices.
 
     Counter = (call, pair lo, pair hi, tag), key = (seed lo, seed hi).
-    Normals of step s: call s >> 1 with tag 0, words (0,1) for even s and
-    (2,3) for odd s, mapped by Box-Muller.  Bates (lam > 0) adds the Poisson
-    count of step s, by inverse CDF of mean lam*dt of the 32-bit word
-    W = (h << 16) | l: h is the high half (even s) or low half (odd s) of
-    word (s >> 1) & 3 of call s >> 3 with tag 1; when h >= T >> 16 (T the
-    count-0 threshold) l is the low 16 bits of word 2 of call s with tag 2,
-    otherwise the count is 0 whatever l is.  A non-zero count takes its jump
-    normal from the Box-Muller cosine output of words (0,1) of that tag-2
-    call.  Returns za, zb, jump_sum, n as [n_steps, n_pairs].
+    Normals of step s: call s >> 1 with tag 0, words (wa, wb) = (0,1) for
+    even s and (2,3) for odd s, mapped by Box-Muller with u1 = (wa + 1/2)
+    2^-32 and the 23-bit angle u2 = (wb >> 9) 2^-23.  Bates (lam > 0) adds
+    the Poisson count of step s, by inverse CDF of mean lam*dt of the 32-bit
+    word W = (h << 23) | l: h is the 9 low bits of wb; when h >= T >> 23 (T
+    the count-0 threshold) l is the 23 high bits of word 2 of call s with
+    tag 2, otherwise the count is 0 whatever l is.  A non-zero count takes
+    its jump normal from the Box-Muller cosine output of words (0,1) of that
+    tag-2 call.  Returns za, zb, jump_sum, n as [n_steps, n_pairs].
     """
     pair_ids = np.asarray(pair_ids, dtype=np.uint64)
     npairs = pair_ids.size
@@ -406,13 +407,12 @@ def philox_draws(seed, pair_ids, n_steps, dt=None, lam=0.0, mu_j=0.0, sigma_j=0.
         za[s], zb[s] = _box_muller(wa, wb)
         if bates:
             mu = lam * dt
-            word = call(s >> 3, TAG_POISSON)[(s >> 1) & 3]
-            h = (word >> np.uint64(16)) if (s & 1) == 0 else (word & np.uint64(0xFFFF))
-            thr_hi = min(poisson_threshold0(mu), M32) >> 16
-            maybe = h >= np.uint64(thr_hi)
+            h = wb & np.uint64((1 << GATE_BITS) - 1)
+            gate = min(poisson_threshold0(mu), M32) >> (32 - GATE_BITS)
+            maybe = h >= np.uint64(gate)
             if np.any(maybe):
                 wj = call(s, TAG_JUMP)
-                full = (h << np.uint64(16)) | (wj[2] & np.uint64(0xFFFF))
+                full = (h << np.uint64(32 - GATE_BITS)) | (wj[2] >> np.uint64(GATE_BITS))
                 n = np.where(maybe, poisson_from_word(full, mu), 0)
                 zj, _ = _box_muller(wj[0], wj[1])
                 js[s] = np.where(n > 0, mu_j * n + sigma_j * np.sqrt(n) * zj, 0.0)

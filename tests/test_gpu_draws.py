@@ -38,7 +38,7 @@ def test_philox_random_counters_match_oracle(ope, ora):
 
 @pytest.mark.parametrize("lam", [0.0, 0.5, 30.0])
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("n_steps", [7, 19])  # the 8-step Poisson blocks: a tail alone; two blocks and a tail
+@pytest.mark.parametrize("n_steps", [7, 19])  # the FP32 8-step blocks: a tail alone; two blocks and an odd tail
 def test_philox_draws_match_oracle(ope, ora, lam, precision, n_steps):
     seed, pair0, n_pairs = 42, 2**32 - 150, 300   # pairs straddle the counter's high word
     za, zb, js, nj = ope.debug_draws(seed, pair0, n_pairs, n_steps, 1.0, lam, -0.1, 0.15, precision)
