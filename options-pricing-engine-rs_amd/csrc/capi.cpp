@@ -240,7 +240,16 @@ int make_desc(const Request& r, McDesc& d) {
         // k3 == k4 (gamma1 == gamma2), so the FP32 path's variance term is
         // sqrt(max(k3, 0)) sqrt(v + v'): the clamp at 0 only ever acts for k3 < 0
         d.qe_sq3 = (float)(kLog2eD * std::sqrt(std::fmax(d.k3, 0.0)));
+        d.qd_m0 = hp.theta * d.one_m_ekt;
+        d.qd_s1 = 0.5 * (hp.sigma * hp.sigma * d.ekt * d.one_m_ekt / hp.kappa);
+        d.qd_s0 = 0.5 * d.qe_c2;
+        d.qd_sq3 = std::sqrt(std::fmax(d.k3, 0.0));
     }
+    d.ed_a0 = drift * dt;
+    d.ed_a1 = -0.5 * dt;
+    d.ed_b0 = hp.kappa * hp.theta * dt;
+    d.ed_c1 = 1.0 - hp.kappa * dt;
+    d.ed_sb = hp.sigma * sqdt;
     d.f_v0 = (float)hp.v0;
     d.eu_a0 = (float)(kLog2eD * drift * dt);
     d.eu_a1 = (float)(-0.5 * kLog2eD * dt);
@@ -773,6 +782,27 @@ int ope_debug_philox(uint64_t seed, const uint32_t* ctr, uint32_t n, uint32_t* o
     HIP_TRY(c.dbg_out.ensure(bytes));
     if (n) HIP_TRY(hipMemcpyAsync(c.dbg_in.p, ctr, bytes, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(launch_debug_philox(seed, c.dbg_in.as<uint4>(), c.dbg_out.as<uint4>(), n, c.stream));
+    if (n) HIP_TRY(hipMemcpyAsync(out, c.dbg_out.p, bytes, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    return OPE_OK;
+}
+
+int ope_debug_fp64_math(int32_t fn, const double* x, uint32_t n, double* out) {
+    if ((!x || !out) && n) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
+    if (fn < 0 || fn > 4) return fail(OPE_ERR_INVALID_ARGUMENT, "unknown function %d", fn);
+    if (fn == 2 || fn == 3)
+        for (uint32_t i = 0; i < n; ++i)
+            if (!(x[i] >= 0.0 && x[i] < 8388608.0 && x[i] == std::floor(x[i])))
+                return fail(OPE_ERR_INVALID_ARGUMENT, "angle %u is not an integer in [0, 2^23)", i);
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    int rc = ensure_init(g_ctx);
+    if (rc) return rc;
+    Context& c = g_ctx;
+    const size_t bytes = (size_t)n * 8;
+    HIP_TRY(c.dbg_in.ensure(bytes));
+    HIP_TRY(c.dbg_out.ensure(bytes));
+    if (n) HIP_TRY(hipMemcpyAsync(c.dbg_in.p, x, bytes, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(launch_debug_fp64_math(fn, c.dbg_in.as<double>(), n, c.dbg_out.as<double>(), c.stream));
     if (n) HIP_TRY(hipMemcpyAsync(out, c.dbg_out.p, bytes, hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
     return OPE_OK;

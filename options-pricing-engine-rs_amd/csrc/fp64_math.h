@@ -1,0 +1,123 @@
+// FP64 elementary functions for the Philox-mode FP64 path kernels, for the
+// argument ranges that path produces.  Each is a few FMAs on top of a
+// hardware approximation or an exact reduction, instead of the general
+// device-library function (whose log and sincospi carry double-double
+// compensation and general range reduction).  Accuracy, checked against the
+// host's libm on the GPU (ope_debug_fp64_math): within 2 ulp.
+//
+// Methods:
+//   rcp_f64     v_rcp_f64 refined by two Newton-Raphson steps, y += y (1 - x y).
+//   log_f64     x = 2^k m with m in [sqrt(1/2), sqrt 2) (frexp, exact);
+//               f = m - 1 exactly (Sterbenz); with s = f / (2 + f),
+//               log(1 + f) = f - (f^2/2 - s (f^2/2 + R)), R = sum 2 s^2j/(2j+1)
+//               (the atanh series; the decomposition of fdlibm's e_log.c),
+//               and k ln2 added as k ln2_hi + k ln2_lo (Cody and Waite).
+//   exp_f64     Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, Taylor
+//               polynomial of e^r to degree 14, scaled by 2^k (ldexp).
+//   sincos_rev23  sin and cos of 2 pi k 2^-23 for a 23-bit integer k: the
+//               nearest quarter turn is taken in integer arithmetic (exact),
+//               the remaining angle (at most pi/4) goes through Taylor
+//               polynomials to degree 17 (sine) and 16 (cosine), and the
+//               quarter turn rotates the pair.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ope {
+namespace {
+
+constexpr double kLn2Hi = 0.6931471806019545;       // ln 2 to 29 significant bits: k ln2_hi is exact
+constexpr double kLn2Lo = -4.2009150726810846e-11;  // ln 2 - ln2_hi
+constexpr double kLog2eD = 1.4426950408889634;
+
+__device__ __forceinline__ double rcp_f64(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+// Natural logarithm of a positive normal x.
+__device__ __forceinline__ double log_f64(double x) {
+    double m = __builtin_amdgcn_frexp_mant(x);  // [1/2, 1)
+    int k = __builtin_amdgcn_frexp_exp(x);
+    const bool lo = m < 0.70710678118654752;
+    m = lo ? m + m : m;  // [sqrt(1/2), sqrt 2)
+    k = lo ? k - 1 : k;
+    const double f = m - 1.0;
+    const double s = f * rcp_f64(2.0 + f);
+    const double z = s * s;
+    double R = 0.09523809523809523;  // 2/21
+    R = fma(R, z, 0.10526315789473684);
+    R = fma(R, z, 0.11764705882352941);
+    R = fma(R, z, 0.13333333333333333);
+    R = fma(R, z, 0.15384615384615385);
+    R = fma(R, z, 0.18181818181818182);
+    R = fma(R, z, 0.2222222222222222);
+    R = fma(R, z, 0.2857142857142857);
+    R = fma(R, z, 0.4);
+    R = fma(R, z, 0.6666666666666666);  // 2/3
+    R = R * z;
+    const double hfsq = 0.5 * f * f;
+    const double kd = (double)k;
+    return fma(kd, kLn2Hi, f - (hfsq - fma(s, hfsq + R, kd * kLn2Lo)));
+}
+
+// e^x for |x| < 700.
+__device__ __forceinline__ double exp_f64(double x) {
+    const double kd = __builtin_rint(x * kLog2eD);
+    const double r = fma(-kd, kLn2Lo, fma(-kd, kLn2Hi, x));
+    double p = 1.1470745597729725e-11;  // 1/14!
+    p = fma(p, r, 1.6059043836821613e-10);
+    p = fma(p, r, 2.08767569878681e-09);
+    p = fma(p, r, 2.505210838544172e-08);
+    p = fma(p, r, 2.755731922398589e-07);
+    p = fma(p, r, 2.7557319223985893e-06);
+    p = fma(p, r, 2.48015873015873e-05);
+    p = fma(p, r, 0.0001984126984126984);
+    p = fma(p, r, 0.001388888888888889);
+    p = fma(p, r, 0.008333333333333333);
+    p = fma(p, r, 0.041666666666666664);
+    p = fma(p, r, 0.16666666666666666);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return __builtin_amdgcn_ldexp(p, (int)kd);
+}
+
+// sin and cos of 2 pi k 2^-23, k in [0, 2^23).
+__device__ __forceinline__ void sincos_rev23(uint32_t k, double& sn, double& cs) {
+    const uint32_t j = (k + (1u << 20)) >> 21;            // nearest quarter turn, 0..4
+    const int32_t dk = (int32_t)k - (int32_t)(j << 21);    // [-2^20, 2^20)
+    const double t = (double)dk * 7.490140565847857e-07;   // pi 2^-22: |t| <= pi/4
+    const double t2 = t * t;
+    double ps = 2.8114572543455206e-15;                    // 1/17!
+    ps = fma(ps, t2, -7.647163731819816e-13);
+    ps = fma(ps, t2, 1.6059043836821613e-10);
+    ps = fma(ps, t2, -2.505210838544172e-08);
+    ps = fma(ps, t2, 2.7557319223985893e-06);
+    ps = fma(ps, t2, -0.0001984126984126984);
+    ps = fma(ps, t2, 0.008333333333333333);
+    ps = fma(ps, t2, -0.16666666666666666);
+    const double s = fma(t * t2, ps, t);
+    double pc = 4.779477332387385e-14;                     // 1/16!
+    pc = fma(pc, t2, -1.1470745597729725e-11);
+    pc = fma(pc, t2, 2.08767569878681e-09);
+    pc = fma(pc, t2, -2.755731922398589e-07);
+    pc = fma(pc, t2, 2.48015873015873e-05);
+    pc = fma(pc, t2, -0.001388888888888889);
+    pc = fma(pc, t2, 0.041666666666666664);
+    pc = fma(pc, t2, -0.5);
+    const double c = fma(t2, pc, 1.0);
+    // rotate by j quarter turns: (s, c) -> (c, -s) -> (-s, -c) -> (-c, s)
+    const uint32_t q = j & 3u;
+    const double a = (q & 1u) ? c : s;
+    const double b = (q & 1u) ? s : c;
+    sn = (q & 2u) ? -a : a;
+    cs = (q == 1u || q == 2u) ? -b : b;
+}
+
+}  // namespace
+}  // namespace ope

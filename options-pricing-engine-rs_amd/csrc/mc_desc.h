@@ -49,6 +49,12 @@ struct alignas(16) McDesc {
     double pois_mu, pois_e0, mu_j, sigma_j;
     double lb2_barrier;            // log2(barrier / spot), FP32-path barrier test
 
+    // ---- FP64 Philox-mode constants (hoisted forms of the same quantities)
+    double ed_a0, ed_a1;           // Euler ln-price: inc = a0 + a1 v + sqrt(v) sqrt(dt) z1
+    double ed_b0, ed_c1, ed_sb;    // Euler variance: v' = max(b0 + c1 v + sqrt(v) sb z2, 0)
+    double qd_m0, qd_s1, qd_s0;    // QE: m = ekt v + m0, s2 / 2 = s1 v + s0
+    double qd_sq3;                 // QE: sqrt(max(K3, 0)) (K3 = K4)
+
     // ---- FP32-path constants (log2(S/S0) state)
     float f_v0;
     float eu_a0, eu_a1, eu_a2;     // dy = a0 + a1 v+ + a2 sqrt(v+) z1

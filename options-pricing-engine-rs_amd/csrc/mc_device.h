@@ -17,6 +17,7 @@
 
 #include <type_traits>
 
+#include "fp64_math.h"
 #include "mc_desc.h"
 
 namespace ope {
@@ -101,10 +102,9 @@ constexpr double kLn2 = 0.6931471805599453;
 // normals up to rounding.
 __device__ __forceinline__ void box_muller_f64(uint32_t w0, uint32_t w1, double& za, double& zb) {
     const double u1 = ((double)w0 + 0.5) * 0x1p-32;
-    const double u2 = (double)(w1 >> kGateBits) * 0x1p-23;
-    const double r = sqrt(-2.0 * log(u1));
+    const double r = sqrt(-2.0 * log_f64(u1));
     double s, c;
-    sincospi(2.0 * u2, &s, &c);
+    sincos_rev23(w1 >> kGateBits, s, c);
     za = r * c;
     zb = r * s;
 }
@@ -197,7 +197,7 @@ struct NoBlockEnd {
 // Poisson uniform; below thr0 >> 23 (probability e^{-lambda dt} up to 2^-9)
 // the count is 0, and only otherwise does the step draw its jump call.
 //
-// DEFER (FP32 European payoffs, whose log-price needs only the sum of the
+// DEFER (European payoffs, whose log-price needs only the sum of the
 // jumps): step() gets jump 0, and a step whose gate bits allow a count
 // only records (step, gate bits) in one of three per-lane slots; the jumps
 // of the slots are drawn after the last step and added to *jsum.  A wave then
@@ -218,12 +218,13 @@ struct NoBlockEnd {
 // replaces the sine and the mixing products, and a2, b2 fold into r.
 template <bool FP32, bool JUMPS, bool DEFER = false, bool MIX = false, class Step, class BlockEnd = NoBlockEnd>
 __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& st, Step&& step,
-                                             float* jsum = nullptr, BlockEnd block_end = BlockEnd{}) {
+                                             typename std::conditional<FP32, float, double>::type* jsum = nullptr,
+                                             BlockEnd block_end = BlockEnd{}) {
     using R = typename std::conditional<FP32, float, double>::type;
-    static_assert(!DEFER || (FP32 && JUMPS), "deferred jumps are an FP32 jump-model mode");
+    static_assert(!DEFER || JUMPS, "deferred jumps need the jump model");
     static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
     uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u, es2 = ~0u, ew2 = 0u;
-    float jd = 0.0f;
+    R jd = 0;
     const float phi = MIX ? in_vgpr(d.eu_phi) : 0.0f;
     const uint32_t gate = d.pois_gate;
     auto one = [&](uint32_t wa, uint32_t wb, uint32_t s) {
@@ -256,7 +257,7 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
                         ew2 = h;
                     } else {
                         uint32_t nn;
-                        jd += jump_of_step<true>(d, st, s, h, nn);
+                        jd += jump_of_step<FP32>(d, st, s, h, nn);
                     }
                 } else {
                     j = jump_of_step<FP32>(d, st, s, h, n);
@@ -301,9 +302,9 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     }
     if constexpr (DEFER) {
         uint32_t nn;
-        if (es0 != ~0u) jd += jump_of_step<true>(d, st, es0, ew0, nn);
-        if (es1 != ~0u) jd += jump_of_step<true>(d, st, es1, ew1, nn);
-        if (es2 != ~0u) jd += jump_of_step<true>(d, st, es2, ew2, nn);
+        if (es0 != ~0u) jd += jump_of_step<FP32>(d, st, es0, ew0, nn);
+        if (es1 != ~0u) jd += jump_of_step<FP32>(d, st, es1, ew1, nn);
+        if (es2 != ~0u) jd += jump_of_step<FP32>(d, st, es2, ew2, nn);
         *jsum += jd;
     }
 }
@@ -541,10 +542,155 @@ struct ReplaySource {
     }
 };
 
+// ------------------------------------------- FP64 path, Philox mode
+// The Philox-mode FP64 kernels need not repeat the reference's evaluation
+// order (their draws are not the reference's; replay mode keeps that order),
+// so they take the FP32 path's algebra in FP64: loop invariants hoisted on
+// the host (McDesc ed_* / qd_*), no division by psi, one erfc per antithetic
+// pair, reciprocals from rcp_f64, and the fast log / exp / sincos of
+// fp64_math.h.  Results agree with the reference-order arithmetic to a few
+// ulp per step.
+
+// Full-truncation Euler for one sibling: v >= 0 on entry (a negative v0
+// enters as 0 with b0 + v0 for its first step, as in the FP32 path).  sa =
+// sqrt(dt) z1 and sb = sigma sqrt(dt) z2 are per-pair (negated for sibling b).
+__device__ __forceinline__ double euler_f64_fast(const McDesc& d, double& v, double sa, double sb, double b0) {
+    const double sv = sqrt(v);
+    const double inc = fma(sv, sa, fma(d.ed_a1, v, d.ed_a0));
+    v = fmax(fma(sv, sb, fma(d.ed_c1, v, b0)), 0.0);
+    return inc;
+}
+
+// The QE variance step of one sibling in FP64, as qe_sib_f32: m, half the
+// conditional variance s2h, t = 2/psi (quadratic branch) or 1 - p
+// (exponential branch), and the factors m r and (b + z)^2.
+struct QeSib64 {
+    double mr, bz2, t;
+    bool ex;
+};
+
+__device__ __forceinline__ QeSib64 qe_sib_f64(const McDesc& d, double v, double z) {
+    QeSib64 o;
+    const double m = fma(d.ekt, v, d.qd_m0), s2h = fma(d.qd_s1, v, d.qd_s0);
+    const double msq = m * m;
+    o.ex = s2h > 0.75 * msq;
+    o.t = fmin(msq * rcp_f64(o.ex ? fma(0.5, msq, s2h) : s2h), 2e12);  // psi floor 1e-12
+    const double opb = o.t + sqrt(fma(o.t, o.t, -o.t));                  // 1 + b^2
+    const double bz = sqrt(opb - 1.0) + z;
+    o.bz2 = bz * bz;
+    o.mr = m * rcp_f64(o.ex ? o.t : opb);
+    return o;
+}
+
+// One QE step of both siblings (mc.rs:256-267, 288-309): returns the ln-price
+// increments (without the jump).  |za| is common to the siblings, so the
+// exponential branch's erfc is evaluated once per pair, and only when a lane
+// of the wave needs it.  K3 = K4, so sqrt(max(K3 v + K4 v', 0)) =
+// sqrt(max(K3, 0)) sqrt(v + v').
+__device__ __forceinline__ void qe_pair_f64_fast(const McDesc& d, bool anti, double& va, double& vb, double za,
+                                                 double zb, double& dya, double& dyb) {
+    const QeSib64 a = qe_sib_f64(d, va, za);
+    QeSib64 b{};
+    if (anti) b = qe_sib_f64(d, vb, -za);
+    double qa = a.bz2, qb = b.bz2;
+    if (a.ex || (anti && b.ex)) {
+        // e = ncdf(-|za|) (math.rs:16-19); 1 - ncdf(z) is e for the sibling
+        // whose z >= 0 and 1 - e for the other; one reciprocal serves both
+        const double e = 0.5 * erfc(fabs(za) * 0.7071067811865476), ome = 1.0 - e;
+        const bool pos = za >= 0.0;
+        const double r = rcp_f64(e * ome);
+        qa = a.ex ? fmax(log_f64(a.t * ((pos ? ome : e) * r)), 0.0) : qa;
+        if (anti) qb = b.ex ? fmax(log_f64(b.t * ((pos ? e : ome) * r)), 0.0) : qb;
+    }
+    const double zs = d.qd_sq3 * zb;
+    const double na = a.mr * qa;
+    dya = fma(sqrt(va + na), zs, fma(d.k2, na, fma(d.k1, va, d.qe_a0)));
+    va = na;
+    if (anti) {
+        const double nb = b.mr * qb;
+        dyb = fma(sqrt(vb + nb), -zs, fma(d.k2, nb, fma(d.k1, vb, d.qe_a0)));
+        vb = nb;
+    } else {
+        dyb = 0.0;
+    }
+}
+
+// Pair simulation, FP64, Philox mode.
+template <int SCHEME, bool JUMPS, int PAYOFF>
+__device__ __forceinline__ void sim_pair_f64_fast(const McDesc& d, const PhiloxSource& src, bool anti,
+                                                  PathState& ea, PathState& eb) {
+    double la = d.ln_spot, lb = d.ln_spot;
+    double va = fmax(d.v0, 0.0), vb = va;
+    double suma = 0.0, sumb = 0.0, jsum = 0.0;
+    double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
+    const uint32_t N = d.n_steps;
+    double b0s = d.v0 < 0.0 ? d.v0 + d.ed_b0 : d.ed_b0;
+    constexpr bool kDefer = JUMPS && PAYOFF == kPayEuropean;
+
+    auto step = [&](double za, double zb, double jump, uint32_t, uint32_t) {
+        double dya, dyb;
+        if (SCHEME == kSchemeEuler) {
+            const double sa = d.sqdt * za;
+            const double sb = d.ed_sb * fma(d.rho, za, d.sqrt_1m_rho2 * zb);
+            dya = euler_f64_fast(d, va, sa, sb, b0s);
+            dyb = euler_f64_fast(d, vb, -sa, -sb, b0s);  // sibling b unused without antithetic pairs
+            b0s = d.ed_b0;
+        } else {
+            qe_pair_f64_fast(d, anti, va, vb, za, zb, dya, dyb);
+        }
+        if (JUMPS && !kDefer) {
+            dya += jump;
+            dyb += jump;
+        }
+        la += dya;
+        lb += dyb;
+        if (PAYOFF == kPayAsian) {
+            suma += exp_f64(la);
+            sumb += exp_f64(lb);
+        }
+        if (PAYOFF == kPayUpAndOut) {
+            mxa = fmax(mxa, la);
+            mxb = fmax(mxb, lb);
+        }
+    };
+    philox_steps<false, JUMPS, kDefer>(d, src.st, step, &jsum);
+    if (kDefer) {
+        la += jsum;
+        lb += jsum;
+    }
+
+    const double nan = __builtin_nan("");
+    auto finish = [&](PathState& e, double ln_s, double v, double sum, double mx) {
+        e.ln_s = ln_s;
+        e.v = v;
+        e.sum = PAYOFF == kPayAsian ? sum : nan;
+        e.max = nan;
+        e.ko = 0;
+        if (PAYOFF == kPayAsian) {
+            e.x = sum / (double)N;
+        } else {
+            e.x = exp(ln_s);
+            if (PAYOFF == kPayUpAndOut) {
+                const double running_max = fmax(d.spot, exp(mx));  // starts at spot (mc.rs:218)
+                e.max = running_max;
+                e.ko = running_max >= d.barrier;
+            }
+        }
+    };
+    finish(ea, la, va, suma, mxa);
+    if (anti) finish(eb, lb, vb, sumb, mxb);
+}
+
 // --------------------------------------------- pair simulation, FP64 path
+// Replay mode runs the reference's expressions in the reference's order
+// (mc.rs:243-277); Philox mode the hoisted form above.
 template <int SCHEME, bool JUMPS, int PAYOFF, class Src>
 __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bool anti, PathState& ea,
                                              PathState& eb) {
+    if constexpr (Src::kPhilox) {
+        sim_pair_f64_fast<SCHEME, JUMPS, PAYOFF>(d, src, anti, ea, eb);
+        return;
+    }
     double la = d.ln_spot, lb = d.ln_spot, va = d.v0, vb = d.v0;
     double suma = 0.0, sumb = 0.0;
     double mxa = -__builtin_huge_val(), mxb = -__builtin_huge_val();
@@ -572,9 +718,7 @@ __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bo
         }
     };
 
-    if constexpr (Src::kPhilox) {
-        philox_steps<false, JUMPS>(d, src.st, step);
-    } else {
+    if constexpr (!Src::kPhilox) {
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
             src.get(i, za, zb, j);

@@ -82,6 +82,28 @@ __global__ void k_debug_draws(const McDesc* __restrict__ desc, uint64_t pair0, u
     }
 }
 
+// The FP64 Philox path's elementary functions (fp64_math.h) on given inputs:
+// fn 0 log_f64, 1 exp_f64, 2 / 3 sine / cosine of sincos_rev23 (x holds the
+// 23-bit integer angle), 4 rcp_f64.
+__global__ void k_debug_fp64_math(int fn, const double* __restrict__ x, uint32_t n, double* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[i];
+    double r;
+    if (fn == 0) {
+        r = log_f64(v);
+    } else if (fn == 1) {
+        r = exp_f64(v);
+    } else if (fn == 2 || fn == 3) {
+        double sn, cs;
+        sincos_rev23((uint32_t)v, sn, cs);
+        r = fn == 2 ? sn : cs;
+    } else {
+        r = rcp_f64(v);
+    }
+    out[i] = r;
+}
+
 template <class Real, int CODE = 0>
 hipError_t launch_fixed(uint32_t code, const TileLaunch& L, hipStream_t s) {
     if constexpr (CODE < (int)kNumCodes) {
@@ -121,6 +143,12 @@ hipError_t launch_debug_philox(uint64_t seed, const uint4* ctr, uint4* out, uint
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_debug_philox, dim3((n + 255) / 256), dim3(256), 0, s, (uint32_t)seed,
                        (uint32_t)(seed >> 32), ctr, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_fp64_math(int fn, const double* x, uint32_t n, double* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_debug_fp64_math, dim3((n + 255) / 256), dim3(256), 0, s, fn, x, n, out);
     return hipGetLastError();
 }
 

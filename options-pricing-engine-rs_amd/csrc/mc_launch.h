@@ -46,6 +46,7 @@ hipError_t launch_reduce(const McDesc* descs, const uint2* jobs, uint32_t n_jobs
                          double2* out, hipStream_t s);
 hipError_t launch_replay(const ReplayLaunch& L, hipStream_t s);
 hipError_t launch_debug_philox(uint64_t seed, const uint4* ctr, uint4* out, uint32_t n, hipStream_t s);
+hipError_t launch_debug_fp64_math(int fn, const double* x, uint32_t n, double* out, hipStream_t s);
 hipError_t launch_debug_draws(const McDesc* desc, uint64_t pair0, uint64_t n_pairs, int fp32, double* za,
                               double* zb, double* jump, uint32_t* n_jumps, hipStream_t s);
 

@@ -140,8 +140,10 @@ def lib():
     L.ope_debug_draws.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, d, d, d, d, C.c_int32,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_uint32)]
+    L.ope_debug_fp64_math.argtypes = [C.c_int32, C.POINTER(C.c_double), C.c_uint32, C.POINTER(C.c_double)]
     for name in ("ope_mc_heston", "ope_mc_bates", "ope_mc_heston_ex", "ope_mc_bates_ex", "ope_mc_surface",
-                 "ope_mc_batch", "ope_get_device_info", "ope_debug_philox", "ope_debug_draws"):
+                 "ope_mc_batch", "ope_get_device_info", "ope_debug_philox", "ope_debug_draws",
+                 "ope_debug_fp64_math"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -423,6 +425,16 @@ def debug_philox(seed: int, ctr: np.ndarray) -> np.ndarray:
     out = np.zeros_like(ctr)
     _check(lib().ope_debug_philox(seed, ctr.ctypes.data_as(C.POINTER(C.c_uint32)), ctr.shape[0],
                                   out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return out
+
+
+def debug_fp64_math(fn: int, x) -> np.ndarray:
+    """The FP64 path's elementary functions on the GPU: fn 0 log, 1 exp,
+    2 / 3 sin / cos of 2 pi x 2^-23 (integer x), 4 reciprocal."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    out = np.zeros_like(x)
+    P = C.POINTER(C.c_double)
+    _check(lib().ope_debug_fp64_math(int(fn), x.ctypes.data_as(P), x.size, out.ctypes.data_as(P)))
     return out
 
 

@@ -144,18 +144,21 @@ def test_barrier_at_or_below_spot_pays_discounted_rebate(ope, precision):
 @pytest.mark.parametrize("scheme", [0, 1])
 @pytest.mark.parametrize("lam", [0.0, 0.5, 30.0])
 def test_barrier_at_infinity_is_european(ope, precision, scheme, lam):
-    # pathwise identical payoffs: bit-identical in FP64; in FP32-path mode the
-    # European kernel sums log-price increments in 8-step FP32 blocks (and,
-    # with jumps, draws the jump sizes after the last step from two per-lane
-    # slots, inline beyond two) while the barrier kernel folds every step,
-    # jump included, into FP64, so only rounding differs.  lam = 30 over 30
-    # steps jumps at most steps, overflowing the slots.
+    # pathwise identical payoffs.  FP64 without jumps: bit-identical.  With
+    # jumps the European kernels draw the jump sizes after the last step (from
+    # three per-lane slots, inline beyond three) and add their sum once, while
+    # the barrier kernel adds each jump in its step; in FP32-path mode the
+    # European kernel also sums log-price increments in 8-step FP32 blocks.
+    # So only rounding differs.  lam = 30 over 30 steps jumps at most steps,
+    # overflowing the slots.
     cfg = ope.McConfig(n_paths=50_000, n_steps=30, seed=9, scheme=scheme)
     jumps = (lam, -0.1, 0.15) if lam > 0 else None
     e = run(ope, precision, ope.Payoff.European(100.0), cfg, hp=BASE, jumps=jumps).result
     u = run(ope, precision, ope.Payoff.UpAndOut(100.0, 1e300), cfg, hp=BASE, jumps=jumps).result
-    if precision == 0:
+    if precision == 0 and lam == 0:
         assert e == u
+    elif precision == 0:
+        assert abs(e.price - u.price) < 1e-9 * e.std_error
     else:
         assert abs(e.price - u.price) < 1e-3 * e.std_error
 
