@@ -429,6 +429,28 @@ def time_mode(run, precision, steps, warmup):
     return prices, ses, wall / steps * 1e3, sum(kms) / len(kms)
 
 
+def barrier_fp64_cost(ope, steps=10):
+    """The QE up-and-out call of the headline workload (1M paths x 365 steps,
+    B=130) in FP32-path mode with and without OPE_FLAG_BARRIER_FP64: kernel
+    time and price of each (the flag decides near-B paths by their FP64
+    path)."""
+    hp = ope.HestonParams(**HP)
+    cfg = ope.McConfig(n_paths=1_000_000, n_steps=365, seed=42, scheme=ope.VarianceScheme.QuadraticExponential)
+    pay = ope.Payoff.UpAndOut(100.0, 130.0, 0.0)
+    out = {}
+    for flag in (False, True):
+        kms, r = [], None
+        for i in range(steps + 2):
+            r = ope.mc_price_ex(100.0, 1.0, 0.03, 0.0, hp, pay, cfg, None, ope.McOptions(1, barrier_fp64=flag))
+            if i >= 2:
+                kms.append(r.kernel_ms)
+        out["on" if flag else "off"] = {"kernel_ms": sum(kms) / len(kms), "price": r.result.price,
+                                        "std_error": r.result.std_error}
+    print(f"# barrier_fp64: off {out['off']['kernel_ms']:.3f} ms, on {out['on']['kernel_ms']:.3f} ms",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def host_info():
     """The host the CPU baseline ran on: CPU model and logical CPU count."""
     model = "unknown"
@@ -616,6 +638,7 @@ def main():
                 resim = len(STRIKES) if extra == "bates_surface" else 1
                 cb["extrapolated_workload_s"] = line[extra]["path_steps"] * resim / cb["value"]
                 line[extra]["cpu_baseline"] = cb
+        line["barrier_fp64_flag"] = barrier_fp64_cost(ope)
         line["analytic_cf"] = analytic_cf(ope, wm, with_cpu=not args.no_cpu)
         line["bsm_iv"] = bsm_iv(ope, wm, with_cpu=not args.no_cpu)
         line["calibration_eval"] = calibration_eval(ope, with_cpu=not args.no_cpu)

@@ -502,6 +502,7 @@ int check_options(const ope_mc_options* o) {
         return fail(OPE_ERR_INVALID_ARGUMENT, "unknown precision %d", o->precision);
     if (o->rng != OPE_RNG_PHILOX && o->rng != OPE_RNG_REPLAY)
         return fail(OPE_ERR_INVALID_ARGUMENT, "unknown rng mode %d", o->rng);
+    if (o->flags & ~OPE_FLAG_BARRIER_FP64) return fail(OPE_ERR_INVALID_ARGUMENT, "unknown flags 0x%x", o->flags);
     return OPE_OK;
 }
 
@@ -589,6 +590,7 @@ int price_single(const Request& r, const ope_mc_options* o, ope_mc_result* out) 
     if (rc) return rc;
     Job job;
     if ((rc = make_desc(r, job.d))) return rc;
+    job.d.flags = o ? o->flags : 0u;
     job.strikes.push_back(r.payoff.strike);
     std::lock_guard<std::mutex> lk(g_ctx.mu);
     if ((rc = ensure_init(g_ctx))) return rc;
@@ -722,6 +724,7 @@ int ope_mc_batch(const ope_mc_request* reqs, uint32_t n, const ope_mc_options* o
         Request r = make_request(q.spot, q.expiry, q.rate, q.div_yield, &q.heston, true, q.lambda, q.mu_j, q.sigma_j,
                                  &q.payoff, &q.cfg);
         if ((rc = make_desc(r, jobs[i].d))) return fail(rc, "request %u: %s", i, g_last_error.c_str());
+        jobs[i].d.flags = opts ? opts->flags : 0u;
         jobs[i].strikes.push_back(q.payoff.strike);
     }
     std::lock_guard<std::mutex> lk(g_ctx.mu);

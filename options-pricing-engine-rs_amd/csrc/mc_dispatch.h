@@ -27,6 +27,29 @@ struct Code {
     static constexpr int kScheme = ((CODE >> 1) / 3) >> 1;
 };
 
+// Waves per SIMD pinned with __launch_bounds__ for the FP32-path up-and-out
+// bodies: their cold FP64 re-simulation near the barrier must not raise the
+// register budget of the FP32 loops, so each keeps the occupancy it has
+// without that code (the rare FP64 pass spills instead).  1: no constraint.
+template <class Real, int CODE>
+constexpr int pinned_waves() {
+    if (!std::is_same<Real, float>::value) return 1;
+    switch (CODE) {
+    case 4: return 8;
+    case 5: return 4;
+    case 10: return 6;
+    case 11: return 3;
+    case 16: return 8;
+    case 17: return 5;
+    case 22: return 8;
+    case 23: return 4;
+    default: return 1;
+    }
+}
+// The mixed FP32 kernel and the FP32 serving kernel, which hold every body.
+constexpr int kPinnedWavesMixedF32 = 3;
+constexpr int kPinnedWavesServerF32 = 4;
+
 template <class Real, int CODE>
 __device__ __forceinline__ void run_code(const McDesc& d, uint32_t tile, const double* strikes, double2* slots,
                                          double2* lds) {

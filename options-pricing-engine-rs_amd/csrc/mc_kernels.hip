@@ -12,7 +12,7 @@ namespace {
 
 // One block per tile; every request in the launch has config code CODE.
 template <class Real, int CODE>
-__global__ __launch_bounds__(kBlock) void k_tiles_fixed(const McDesc* __restrict__ descs,
+__global__ __launch_bounds__(kBlock, (pinned_waves<Real, CODE>())) void k_tiles_fixed(const McDesc* __restrict__ descs,
                                                         const uint2* __restrict__ tiles,
                                                         const double* __restrict__ strikes,
                                                         double2* __restrict__ slots) {
@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kBlock) void k_tiles_fixed(const McDesc* __restrict
 
 // One block per tile; each tile's request picks its body (uniform branch).
 template <class Real>
-__global__ __launch_bounds__(kBlock) void k_tiles_mixed(const McDesc* __restrict__ descs,
+__global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinnedWavesMixedF32 : 1)) void k_tiles_mixed(const McDesc* __restrict__ descs,
                                                         const uint2* __restrict__ tiles,
                                                         const double* __restrict__ strikes,
                                                         double2* __restrict__ slots) {

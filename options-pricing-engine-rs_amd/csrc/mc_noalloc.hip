@@ -132,6 +132,7 @@ extern "C" int ope_mc_bates_noalloc(double spot, double expiry, double rate, dou
     std::memset(&c, 0, sizeof c);
     const Request r = make_request(spot, expiry, rate, div_yield, params, true, lambda, mu_j, sigma_j, payoff, cfg);
     if ((rc = make_desc(r, c.d))) return rc;
+    c.d.flags = opts ? opts->flags : 0u;
     if ((rc = set_single_tiling(c.d))) return rc;
     if (c.d.n_tiles > kMaxTiles)
         return fail(OPE_ERR_INVALID_ARGUMENT,

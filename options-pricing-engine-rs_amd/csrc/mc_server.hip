@@ -90,7 +90,7 @@ __device__ void sentinel(const SrvLaunch& L, SrvMirror* mirror) {
 }
 
 template <class Real>
-__global__ __launch_bounds__(kBlock) void k_server(SrvLaunch L, SrvMirror* mirror) {
+__global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinnedWavesServerF32 : 1)) void k_server(SrvLaunch L, SrvMirror* mirror) {
     __shared__ double2 lds[kMaxStrikes * 4];
     __shared__ uint32_t s_rec[kRecWords];
     __shared__ SrvItem s_item;

@@ -38,6 +38,8 @@ OPE_ERR_CANCELLED = -8
 OPE_ERR_DEADLINE = -9
 OPE_ERR_TIMEOUT = -10
 
+FLAG_BARRIER_FP64 = 1
+
 SERVER_STOP_ALL_PATHS = 0
 SERVER_STOP_TARGET = 1
 SERVER_STOP_DEADLINE = 2
@@ -89,7 +91,7 @@ class _Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("rng", C.c_int32), ("replay", C.POINTER(_Replay)),
                 ("path_payoffs", C.POINTER(C.c_double)), ("path_states", C.POINTER(C.c_double)),
                 ("stats", C.POINTER(_Stats)), ("device_ms", C.POINTER(C.c_double)),
-                ("kernel_ms", C.POINTER(C.c_double))]
+                ("kernel_ms", C.POINTER(C.c_double)), ("flags", C.c_uint32), ("reserved_", C.c_uint32)]
 
 
 class _Request(C.Structure):
@@ -258,10 +260,11 @@ class McStats:
 class McOptions:
     """Extension options (precision, replay draws, per-path outputs)."""
 
-    def __init__(self, precision=PRECISION_FP64, replay=None, want_paths=False):
+    def __init__(self, precision=PRECISION_FP64, replay=None, want_paths=False, barrier_fp64=False):
         self.precision = precision
         self.replay = replay  # (za, zb, jump_sum) as [n_steps, n_pairs] arrays, jump_sum may be None
         self.want_paths = want_paths
+        self.barrier_fp64 = barrier_fp64  # OPE_FLAG_BARRIER_FP64
 
 
 class _OptionsHolder:
@@ -272,6 +275,7 @@ class _OptionsHolder:
         self.c = _Options()
         self.c.precision = o.precision
         self.c.rng = RNG_REPLAY if o.replay is not None else RNG_PHILOX
+        self.c.flags = FLAG_BARRIER_FP64 if getattr(o, "barrier_fp64", False) else 0
         self.stats = (_Stats * n_results)()
         self.c.stats = C.cast(self.stats, C.POINTER(_Stats))
         self.ms = C.c_double(0.0)
