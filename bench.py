@@ -395,7 +395,7 @@ def calibration_eval(ope, steps=5, with_cpu=True):
         out[name] = entry
         print(f"# calibration {name}: {wall * 1e3:.2f} ms per evaluation ({entry['prices_per_eval']} prices)",
               file=sys.stderr, flush=True)
-    out["workload"] = ("residuals, FD Jacobian, J'J and J'r of 328 quotes (41 strikes 60..140 x 8 expiries "
+    out["workload"] = ("residuals, Jacobian (central differences; Bates jump columns dP/dparam / vega as the reference's AD), J'J and J'r of 328 quotes (41 strikes 60..140 x 8 expiries "
                        "0.1..2, implied vols of the Heston / Bates prices at v0=theta=0.04, kappa=1.5, sigma=0.5, "
                        "rho=-0.7 [, lambda=0.5, mu_j=-0.1, sigma_j=0.15]) at a parameter vector away from it")
     return out

@@ -1,7 +1,8 @@
 /*
  * ope_calib.h -- the evaluations of one Levenberg-Marquardt calibration step
  * on the GPU: implied-volatility residuals of a quote set under a Heston or
- * Bates parameter vector, their finite-difference Jacobian, and J'J / J'r.
+ * Bates parameter vector, their Jacobian (central differences, and for the
+ * Bates jump columns the exact price derivative over vega), and J'J / J'r.
  *
  * They replace the private helpers the reference's calibrate_* drivers call
  * every iteration (no public counterpart):
@@ -18,9 +19,13 @@
  * of the analytic pricer, and all their implied volatilities through one
  * launch of the solver.
  *
- * Difference from the reference: for Bates it takes the three jump columns
- * from forward-mode AD when every quote's solver succeeds
- * (calibration.rs:620-634, ad.rs); here every column is differenced.
+ * Bates jump columns (lambda, mu_j, sigma_j) are the reference's AD columns
+ * (calibration.rs:620-636, 186-198): dPrice/dparam of every quote, integrated
+ * alongside the price on the price's own adaptive panels (the derivative of
+ * the reference's forward-mode-AD quadrature, ad.rs:609-731, 882), divided by
+ * the BSM vega at the model's implied vol (calibration.rs:593-605).  When a
+ * quote's implied vol is not found or its vega is below 1e-10, the whole
+ * column is differenced instead (calibration.rs:622-632).
  *
  * params: the parameter vector in CalibModel::to_vec order (calibration.rs:
  * 105, 145-148): v0, kappa, theta, sigma, rho [, lambda, mu_j, sigma_j].

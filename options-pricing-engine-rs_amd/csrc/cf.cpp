@@ -15,7 +15,7 @@
 #include "engine.h"
 
 namespace ope {
-hipError_t launch_cf(const CfDesc* opts, uint32_t n, double* out, CfStats* stats, hipStream_t s);
+hipError_t launch_cf(const CfDesc* opts, uint32_t n, double* out, CfStats* stats, double* dout, hipStream_t s);
 hipError_t launch_bsm(const ope_option_contract* c, uint32_t n, double* price, ope_pricing_result* greeks,
                       hipStream_t s);
 hipError_t launch_iv(const ope_option_contract* c, const double* mkt, uint32_t n, double* vol, uint8_t* found,
@@ -75,6 +75,8 @@ int make_cf_desc(const ope_option_contract& k, const ope_bates_params& bp, bool 
     d.sigma_j = bates ? bp.sigma_j : 0.0;
     d.bates = bates ? 1 : 0;
     d.put = k.opt_type == OPE_PUT ? 1 : 0;
+    d.deriv = 0;
+    d.pad_ = 0;
     return OPE_OK;
 }
 
@@ -98,7 +100,10 @@ int elapsed(CfContext& c, double* kernel_ms) {
     return OPE_OK;
 }
 
-int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, double* kernel_ms) {
+// Prices of descs (and, with dout, 3 jump-parameter derivatives per option,
+// non-zero for Bates options with the deriv flag) in one k_cf launch.
+int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, double* kernel_ms,
+           double* dout = nullptr) {
     const size_t n = descs.size();
     if (n == 0) return OPE_OK;
     if (n > kMaxOptions) return fail(OPE_ERR_INVALID_ARGUMENT, "more than %zu options in one call", kMaxOptions);
@@ -106,16 +111,19 @@ int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, doub
     if (int rc = cf_init(g_cf)) return rc;
     CfContext& c = g_cf;
     const size_t b_desc = sizeof(CfDesc) * n, b_out = sizeof(double) * n, b_stats = sizeof(CfStats) * n;
-    if (int rc = ensure_buf(c, b_desc + b_out + b_stats)) return rc;
+    const size_t b_dout = dout ? 3 * sizeof(double) * n : 0;
+    if (int rc = ensure_buf(c, b_desc + b_out + b_stats + b_dout)) return rc;
     char* base = static_cast<char*>(c.buf);
     CfDesc* d_desc = reinterpret_cast<CfDesc*>(base);
     double* d_out = reinterpret_cast<double*>(base + b_desc);
     CfStats* d_stats = reinterpret_cast<CfStats*>(base + b_desc + b_out);
+    double* d_dout = dout ? reinterpret_cast<double*>(base + b_desc + b_out + b_stats) : nullptr;
     HIP_TRY(hipMemcpyAsync(d_desc, descs.data(), b_desc, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(hipEventRecord(c.ev0, c.stream));
-    HIP_TRY(launch_cf(d_desc, (uint32_t)n, d_out, panels ? d_stats : nullptr, c.stream));
+    HIP_TRY(launch_cf(d_desc, (uint32_t)n, d_out, panels ? d_stats : nullptr, d_dout, c.stream));
     HIP_TRY(hipEventRecord(c.ev1, c.stream));
     HIP_TRY(hipMemcpyAsync(out, d_out, b_out, hipMemcpyDeviceToHost, c.stream));
+    if (dout) HIP_TRY(hipMemcpyAsync(dout, d_dout, b_dout, hipMemcpyDeviceToHost, c.stream));
     std::vector<CfStats> st;
     if (panels) {
         st.resize(n);
@@ -259,24 +267,49 @@ constexpr double kCalBump[8] = {1e-4, 1e-3, 1e-4, 1e-3, 1e-4, 1e-4, 1e-4, 1e-4};
 constexpr double kCalLo[8] = {1e-8, 1e-6, 1e-8, 1e-6, -0.9999, 0.0, -2.0, 1e-6};
 constexpr double kCalHi[8] = {5.0, 50.0, 5.0, 10.0, 0.9999, 10.0, 2.0, 2.0};
 
-// Residuals (and, with jac, the Jacobian) of one calibration evaluation:
-// every price in one k_cf launch, every implied volatility in one k_iv launch.
+// Residuals (and, with jac, the Jacobian) of one calibration evaluation
+// (calibration.rs:573-587, 613-652).  Every price in one k_cf launch, every
+// implied volatility in one k_iv launch.  Columns are central differences
+// with the bump points clamped to the bounds, except Bates' three jump
+// columns, which the reference takes from forward-mode AD (calibration.rs:
+// 186-199, 617-634): d(price)/d(param) integrated beside the base price
+// (k_cf's derivative pass) divided by the BSM vega at the model's implied
+// vol; a column falls back to differences when any quote's solver finds no
+// volatility or its vega is below 1e-10, as the reference's does.
 int calib_eval(const ope_calib_quote* quotes, size_t n, int32_t model, const double* params, double* res,
                double* jac) {
     if (model != OPE_CF_HESTON && model != OPE_CF_BATES)
         return fail(OPE_ERR_INVALID_ARGUMENT, "unknown model %d", model);
-    const int dim = model == OPE_CF_BATES ? 8 : 5;
+    const bool bates = model == OPE_CF_BATES;
+    const int dim = bates ? 8 : 5;
     for (int c = 0; c < dim; ++c)
         if (!std::isfinite(params[c])) return fail(OPE_ERR_INVALID_ARGUMENT, "non-finite parameter %d", c);
-    // parameter vectors: 0 = base (residuals), then (up, down) per column
-    std::vector<std::array<double, 8>> vecs;
-    std::vector<double> dist;
+    std::vector<double> iv_mkt(n);
+    for (size_t i = 0; i < n; ++i) {
+        iv_mkt[i] = quotes[i].iv_market;
+        if (!std::isfinite(quotes[i].iv_market) || !std::isfinite(quotes[i].weight))
+            return fail(OPE_ERR_INVALID_ARGUMENT, "quote %zu: non-finite iv_market or weight", i);
+    }
     std::array<double, 8> base{};
     for (int c = 0; c < dim; ++c) base[c] = params[c];
-    if (res) vecs.push_back(base);
-    std::vector<int> col_of;  // for each pair of vectors, its column
-    if (jac)
-        for (int c = 0; c < dim; ++c) {
+    const bool ad = jac && bates;          // AD jump columns need the base point
+    const bool with_base = res || ad;
+    const int fd_cols = ad ? 5 : dim;      // columns differenced up front
+
+    // Prices and implied vols of the base point (optional) and of the (up,
+    // down) bump vectors of the given columns, in one k_cf and one k_iv
+    // launch.  r(v, i) is the residual of quote i under vector v.
+    struct Eval {
+        std::vector<int> cols;
+        std::vector<double> dist, px, iv, dpx;
+        std::vector<uint8_t> found;
+        size_t v0 = 0;
+    };
+    auto evaluate = [&](bool want_base, const std::vector<int>& cols, Eval& e) -> int {
+        std::vector<std::array<double, 8>> vecs;
+        if (want_base) vecs.push_back(base);
+        e.v0 = want_base ? 1 : 0;
+        for (int c : cols) {
             std::array<double, 8> vu = base, vd = base;
             vu[c] = std::fmin(base[c] + kCalBump[c], kCalHi[c]);
             vd[c] = std::fmax(base[c] - kCalBump[c], kCalLo[c]);
@@ -284,48 +317,83 @@ int calib_eval(const ope_calib_quote* quotes, size_t n, int32_t model, const dou
             if (h <= 0.0) continue;  // pinned against both bounds: zero column
             vecs.push_back(vu);
             vecs.push_back(vd);
-            dist.push_back(h);
-            col_of.push_back(c);
+            e.dist.push_back(h);
+            e.cols.push_back(c);
         }
-    const size_t nv = vecs.size();
-    if (n > kMaxOptions / std::max<size_t>(nv, 1)) return fail(OPE_ERR_INVALID_ARGUMENT, "too many quotes");
-    std::vector<CfDesc> d(nv * n);
-    std::vector<ope_option_contract> k(nv * n);
-    std::vector<double> iv_mkt(n);
-    for (size_t i = 0; i < n; ++i) {
-        iv_mkt[i] = quotes[i].iv_market;
-        if (!std::isfinite(quotes[i].iv_market) || !std::isfinite(quotes[i].weight))
-            return fail(OPE_ERR_INVALID_ARGUMENT, "quote %zu: non-finite iv_market or weight", i);
-    }
-    for (size_t v = 0; v < nv; ++v) {
-        ope_bates_params bp{};
-        bp.heston = ope_heston_params{vecs[v][0], vecs[v][1], vecs[v][2], vecs[v][3], vecs[v][4]};
-        bp.lambda = vecs[v][5];
-        bp.mu_j = vecs[v][6];
-        bp.sigma_j = vecs[v][7];
-        for (size_t i = 0; i < n; ++i) {
-            if (int rc = make_cf_desc(quotes[i].contract, bp, model == OPE_CF_BATES, d[v * n + i]))
-                return fail(rc, "quote %zu: %s", i, g_last_error.c_str());
-            k[v * n + i] = quotes[i].contract;
+        const size_t nv = vecs.size();
+        if (nv == 0) return OPE_OK;
+        if (n > kMaxOptions / nv) return fail(OPE_ERR_INVALID_ARGUMENT, "too many quotes");
+        std::vector<CfDesc> d(nv * n);
+        std::vector<ope_option_contract> k(nv * n);
+        for (size_t v = 0; v < nv; ++v) {
+            ope_bates_params bp{};
+            bp.heston = ope_heston_params{vecs[v][0], vecs[v][1], vecs[v][2], vecs[v][3], vecs[v][4]};
+            bp.lambda = vecs[v][5];
+            bp.mu_j = vecs[v][6];
+            bp.sigma_j = vecs[v][7];
+            for (size_t i = 0; i < n; ++i) {
+                if (int rc = make_cf_desc(quotes[i].contract, bp, bates, d[v * n + i]))
+                    return fail(rc, "quote %zu: %s", i, g_last_error.c_str());
+                d[v * n + i].deriv = (ad && want_base && v == 0) ? 1 : 0;
+                k[v * n + i] = quotes[i].contract;
+            }
         }
-    }
-    std::vector<double> px(nv * n), iv(nv * n);
-    std::vector<uint8_t> found(nv * n);
-    if (int rc = run_cf(d, px.data(), nullptr, nullptr)) return rc;
-    if (int rc = run_iv(k.data(), px.data(), nv * n, iv.data(), found.data(), nullptr)) return rc;
+        e.px.resize(nv * n);
+        e.iv.resize(nv * n);
+        e.found.resize(nv * n);
+        if (ad && want_base) e.dpx.resize(3 * nv * n);
+        if (int rc = run_cf(d, e.px.data(), nullptr, nullptr, e.dpx.empty() ? nullptr : e.dpx.data())) return rc;
+        return run_iv(k.data(), e.px.data(), nv * n, e.iv.data(), e.found.data(), nullptr);
+    };
     // calibration.rs:573-581: iv - iv_market, or 0 where no volatility is found
-    auto r = [&](size_t v, size_t i) { return found[v * n + i] ? iv[v * n + i] - iv_mkt[i] : 0.0; };
-    size_t v0 = 0;
-    if (res) {
-        for (size_t i = 0; i < n; ++i) res[i] = r(0, i);
-        v0 = 1;
-    }
-    if (jac) {
-        for (size_t i = 0; i < n * (size_t)dim; ++i) jac[i] = 0.0;
-        for (size_t p = 0; p < col_of.size(); ++p) {
-            const size_t vu = v0 + 2 * p, vd = vu + 1;
-            for (size_t i = 0; i < n; ++i) jac[i * dim + col_of[p]] = (r(vu, i) - r(vd, i)) / dist[p];
+    auto r = [&](const Eval& e, size_t v, size_t i) {
+        return e.found[v * n + i] ? e.iv[v * n + i] - iv_mkt[i] : 0.0;
+    };
+    auto fd_fill = [&](const Eval& e) {
+        for (size_t p = 0; p < e.cols.size(); ++p) {
+            const size_t vu = e.v0 + 2 * p, vd = vu + 1;
+            for (size_t i = 0; i < n; ++i) jac[i * dim + e.cols[p]] = (r(e, vu, i) - r(e, vd, i)) / e.dist[p];
         }
+    };
+
+    std::vector<int> cols;
+    if (jac)
+        for (int c = 0; c < fd_cols; ++c) cols.push_back(c);
+    Eval e;
+    if (int rc = evaluate(with_base, cols, e)) return rc;
+    if (res)
+        for (size_t i = 0; i < n; ++i) res[i] = r(e, 0, i);
+    if (!jac) return OPE_OK;
+    for (size_t i = 0; i < n * (size_t)dim; ++i) jac[i] = 0.0;
+    fd_fill(e);
+    if (!ad) return OPE_OK;
+
+    // calibration.rs:601-609 model_iv_and_vega: BSM vega at the implied vol of
+    // the model price (bsm.rs:33, per unit of volatility)
+    std::vector<ope_option_contract> kv(n);
+    bool all_found = true;
+    for (size_t i = 0; i < n; ++i) {
+        kv[i] = quotes[i].contract;
+        kv[i].vol = e.found[i] ? e.iv[i] : 0.2;
+        all_found = all_found && e.found[i];
+    }
+    std::vector<ope_pricing_result> g(n);
+    if (all_found)
+        if (int rc = run_bsm(kv.data(), n, nullptr, g.data(), nullptr)) return rc;
+    std::vector<int> fallback;
+    for (int c = 5; c < 8; ++c) {
+        bool ok = all_found;
+        for (size_t i = 0; ok && i < n; ++i) ok = std::fabs(g[i].vega) >= 1e-10;
+        if (!ok) {
+            fallback.push_back(c);
+            continue;
+        }
+        for (size_t i = 0; i < n; ++i) jac[i * dim + c] = e.dpx[3 * i + (c - 5)] / g[i].vega;
+    }
+    if (!fallback.empty()) {
+        Eval f;
+        if (int rc = evaluate(false, fallback, f)) return rc;
+        fd_fill(f);
     }
     return OPE_OK;
 }

@@ -12,6 +12,8 @@ struct CfDesc {
     double lambda, mu_j, sigma_j;
     int32_t bates;  // 1: Bates (Heston CF x jump CF), 0: Heston
     int32_t put;    // 1: put by parity from the call
+    int32_t deriv;  // Bates in a derivative launch: also d(price)/d(lambda, mu_j, sigma_j)
+    int32_t pad_;
 };
 
 // Adaptive Gauss-Kronrod limits of the reference (heston.rs:193-195, 228).

@@ -91,6 +91,29 @@ __device__ cpx jump_cf(cpx phi, const CfDesc& p) {
     return cexp_(cscale(inner, p.lambda * p.expiry));
 }
 
+// d ln J(phi) / d(lambda, mu_j, sigma_j) of the jump factor above, J =
+// exp(lambda T (E - 1 - i phi k)), E = exp(i phi mu_j - sigma_j^2 phi^2 / 2),
+// k = e^(mu_j + sigma_j^2/2) - 1:
+//   lambda:  T (E - 1 - i phi k)
+//   mu_j:    lambda T i phi (E - e^(mu_j + sigma_j^2/2))
+//   sigma_j: -lambda T sigma_j (phi^2 E + i phi e^(mu_j + sigma_j^2/2))
+// The integrands' derivatives are Re(z g) for the integrand z: only J
+// depends on the jump parameters, and the P1 normaliser's factor J(-i) is 1
+// with zero derivative (E(-i) = e^(mu_j + sigma_j^2/2)).  These are the
+// quantities the reference's forward-mode pass (ad.rs:218-234, 609-630)
+// propagates through the same expressions.
+__device__ void jump_dlog(cpx phi, const CfDesc& p, cpx g[3]) {
+    const cpx I = C(0.0, 1.0);
+    const double ec = exp(p.mu_j + 0.5 * p.sigma_j * p.sigma_j);
+    const cpx E = cexp_(csub(cscale(cmul(phi, I), p.mu_j),
+                             cscale(cscale(cmul(cscale(phi, 0.5), phi), p.sigma_j), p.sigma_j)));
+    const cpx iphi = cmul(I, phi);
+    const cpx inner = csub(cradd(E, -1.0), cscale(iphi, ec - 1.0));
+    g[0] = cscale(inner, p.expiry);
+    g[1] = cscale(cmul(iphi, cradd(E, -ec)), p.lambda * p.expiry);
+    g[2] = cscale(cadd(cmul(cmul(phi, phi), E), cscale(iphi, ec)), -p.lambda * p.expiry * p.sigma_j);
+}
+
 __device__ __forceinline__ cpx model_cf(cpx phi, const CfDesc& p) {
     cpx cf = stable_cf(phi, p);
     if (p.bates) cf = cmul(cf, jump_cf(phi, p));
@@ -98,7 +121,13 @@ __device__ __forceinline__ cpx model_cf(cpx phi, const CfDesc& p) {
 }
 
 // heston.rs:101-109 / bates.rs:65-73, then the map of heston.rs:220-229.
-__device__ double mapped(double t, const CfDesc& p, double x, bool is_p1, cpx norm) {
+// With DERIV (Bates), dv[k] receives the derivative of the mapped integrand
+// with respect to (lambda, mu_j, sigma_j), 0 where it or the value is not
+// finite (ad.rs:673-684); the value keeps the pricer's own guard, so the
+// quadrature takes the same bisection decisions as the price alone.
+template <bool DERIV>
+__device__ double mapped(double t, const CfDesc& p, double x, bool is_p1, cpx norm, double* dv) {
+    if (DERIV) dv[0] = dv[1] = dv[2] = 0.0;
     if (t <= 0.0) return 0.0;
     const double u = (1.0 - t) / t;
     if (u < 1e-12) return 0.0;
@@ -106,8 +135,18 @@ __device__ double mapped(double t, const CfDesc& p, double x, bool is_p1, cpx no
     cpx cf = model_cf(phi, p);
     if (is_p1) cf = cdiv(cf, norm);
     const cpx num = cmul(cexp_(C(0.0, u * x)), cf);
-    const double v = cdiv(num, C(0.0, u)).re / (t * t);
-    return isfinite(v) ? v : 0.0;
+    const cpx z = cdiv(num, C(0.0, u));
+    const double v = z.re / (t * t);
+    if (!isfinite(v)) return 0.0;
+    if (DERIV) {
+        cpx g[3];
+        jump_dlog(phi, p, g);
+        for (int k = 0; k < 3; ++k) {
+            const double dk = cmul(z, g[k]).re / (t * t);
+            dv[k] = isfinite(dk) ? dk : 0.0;
+        }
+    }
+    return v;
 }
 
 // heston.rs:28-59
@@ -157,13 +196,29 @@ struct Panels {  // one adaptive integral's live panels, in the reference's orde
     double b[kCfMaxPanels + 2];
 };
 
+// The derivative integrals' K15 sums per live panel (DERIV).
+struct PanelsD {
+    double k[3][kCfMaxPanels + 2];
+};
+
+// DERIV: Bates options also integrate the three jump-parameter derivatives of
+// the integrands on the same panels, the value's error driving the
+// bisection (ad.rs:639-670), and dout receives d(price)/d(lambda, mu_j,
+// sigma_j) per option (ad.rs:711-730).  Options without the deriv flag
+// (and Heston options) in a DERIV launch get zeros.
+template <bool DERIV>
 __global__ __launch_bounds__(64) void k_cf(const CfDesc* __restrict__ opts, double* __restrict__ out,
-                                           CfStats* __restrict__ stats) {
+                                           CfStats* __restrict__ stats, double* __restrict__ dout) {
     __shared__ Panels P[2];
     __shared__ double fv[2][2][15];
     __shared__ double s_total[2], s_err[2], s_kw[2], s_ew[2], s_aw[2], s_m[2], s_bw[2];
     __shared__ double s_kc[2][2], s_ec[2][2];
     __shared__ int s_n[2];
+    // derivative state (sized 1 without DERIV)
+    constexpr int ND = DERIV ? 2 : 1;
+    __shared__ PanelsD PD[ND];
+    __shared__ double fd[ND][2][3][15];
+    __shared__ double s_totd[ND][3], s_kdw[ND][3], s_kdc[ND][2][3];
 
     const CfDesc p = opts[blockIdx.x];
     const int lane = threadIdx.x;
@@ -174,6 +229,8 @@ __global__ __launch_bounds__(64) void k_cf(const CfDesc* __restrict__ opts, doub
     // CF(-i), the P1 normaliser (heston.rs:91, bates.rs:55)
     const cpx norm = is_p1 ? model_cf(C(0.0, -1.0), p) : C(1.0, 0.0);
     Panels& Q = P[half];
+    const bool deriv = DERIV && p.bates != 0 && p.deriv != 0;  // uniform per block
+    const int hd = DERIV ? half : 0;
 
     // Round 0 evaluates the first panel, [0, 1], as a lone left child; every
     // later round splits the panel with the largest error into two children.
@@ -215,6 +272,11 @@ __global__ __launch_bounds__(64) void k_cf(const CfDesc* __restrict__ opts, doub
                     const int w = bi, last = np - 1;
                     s_ew[half] = Q.e[w];
                     s_kw[half] = Q.k[w];
+                    if (deriv)
+                        for (int k = 0; k < 3; ++k) {
+                            s_kdw[hd][k] = PD[hd].k[k][w];
+                            PD[hd].k[k][w] = PD[hd].k[k][last];
+                        }
                     const double aw = Q.a[w], bw = Q.b[w];
                     Q.e[w] = Q.e[last];  // swap_remove
                     Q.k[w] = Q.k[last];
@@ -234,17 +296,44 @@ __global__ __launch_bounds__(64) void k_cf(const CfDesc* __restrict__ opts, doub
             const double a = c ? s_m[half] : s_aw[half];
             const double b = c ? s_bw[half] : s_m[half];
             const double cc = 0.5 * (a + b), h = 0.5 * (b - a);
-            fv[half][c][i] = mapped(cc + h * kGkNodes[i], p, x, is_p1, norm);
+            if (deriv) {
+                double dv[3];
+                fv[half][c][i] = mapped<true>(cc + h * kGkNodes[i], p, x, is_p1, norm, dv);
+                for (int k = 0; k < 3; ++k) fd[hd][c][k][i] = dv[k];
+            } else {
+                fv[half][c][i] = mapped<false>(cc + h * kGkNodes[i], p, x, is_p1, norm, nullptr);
+            }
         }
         __syncthreads();
         if (act && hl < children) {
             const double a = hl ? s_m[half] : s_aw[half];
             const double b = hl ? s_bw[half] : s_m[half];
             panel_sums(fv[half][hl], a, b, s_kc[half][hl], s_ec[half][hl]);
+            if (deriv) {  // ad.rs:639-647: K15 of the derivative values, times h
+                const double h = 0.5 * (b - a);
+                for (int k = 0; k < 3; ++k) {
+                    double kd = 0.0;
+                    for (int i = 0; i < 15; ++i) kd += kGkWeights[i] * fd[hd][hl][k][i];
+                    s_kdc[hd][hl][k] = kd * h;
+                }
+            }
         }
         __syncthreads();
         if (act && hl == 0) {
             const double kl = s_kc[half][0], el = s_ec[half][0];
+            if (deriv)
+                for (int k = 0; k < 3; ++k) {
+                    const double dl = s_kdc[hd][0][k];
+                    if (round == 0) {
+                        PD[hd].k[k][0] = dl;
+                        s_totd[hd][k] = dl;
+                    } else {
+                        const double dr = s_kdc[hd][1][k];
+                        s_totd[hd][k] += dl + dr - s_kdw[hd][k];
+                        PD[hd].k[k][s_n[half]] = dl;
+                        PD[hd].k[k][s_n[half] + 1] = dr;
+                    }
+                }
             if (round == 0) {
                 Q.e[0] = el;
                 Q.k[0] = kl;
@@ -284,14 +373,26 @@ __global__ __launch_bounds__(64) void k_cf(const CfDesc* __restrict__ opts, doub
         if (p.put) v = v - p.spot * eq + p.strike * er;
         out[blockIdx.x] = v;
         if (stats) stats[blockIdx.x] = CfStats{(uint32_t)s_n[0], (uint32_t)s_n[1]};
+        if (DERIV) {
+            // ad.rs:711-730: seq dP1 - ker dP2, the same for a put (parity terms
+            // do not depend on the jump parameters)
+            for (int k = 0; k < 3; ++k) {
+                double dk = 0.0;
+                if (deriv) dk = p.spot * eq * (s_totd[0][k] / PI) - p.strike * er * (s_totd[1][k] / PI);
+                dout[3 * blockIdx.x + k] = dk;
+            }
+        }
     }
 }
 
 }  // namespace
 
-hipError_t launch_cf(const CfDesc* opts, uint32_t n, double* out, CfStats* stats, hipStream_t s) {
+hipError_t launch_cf(const CfDesc* opts, uint32_t n, double* out, CfStats* stats, double* dout, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cf, dim3(n), dim3(64), 0, s, opts, out, stats);
+    if (dout)
+        hipLaunchKernelGGL(k_cf<true>, dim3(n), dim3(64), 0, s, opts, out, stats, dout);
+    else
+        hipLaunchKernelGGL(k_cf<false>, dim3(n), dim3(64), 0, s, opts, out, stats, dout);
     return hipGetLastError();
 }
 

@@ -98,6 +98,8 @@ def lib():
                                 C.c_uint64, _dp, _dp, _dp, _dp, _dp]
     L.ora_heston_price.restype = C.c_double
     L.ora_heston_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston), C.c_int]
+    L.ora_bates_jump_sens.restype = None
+    L.ora_bates_jump_sens.argtypes = [C.c_double] * 5 + [C.POINTER(Heston)] + [C.c_double] * 3 + [C.c_int, _dp]
     L.ora_bates_price.restype = C.c_double
     L.ora_bates_price.argtypes = [C.c_double] * 5 + [C.POINTER(Heston)] + [C.c_double] * 3 + [C.c_int]
     L.ora_bsm_greeks.restype = None
@@ -195,6 +197,13 @@ def bates_price(s, k, t, r, q, hp, lam, mu_j, sigma_j, opt_type=CALL):
 
 
 GREEK_NAMES = ("price", "delta", "gamma", "vega", "theta", "rho", "vanna", "volga")
+
+
+def bates_jump_sens(s, k, t, r, q, hp, lam, mu_j, sigma_j, opt_type=CALL):
+    """ad.rs:882: (price, d/d lambda, d/d mu_j, d/d sigma_j) of the Bates price."""
+    out = np.zeros(4)
+    lib().ora_bates_jump_sens(s, k, t, r, q, C.byref(hp), lam, mu_j, sigma_j, opt_type, out)
+    return out
 
 
 def cf_greeks(s, k, t, r, q, hp, jumps=None, opt_type=CALL):

@@ -8,6 +8,13 @@
  *                  stable_cf (Albrecher form) 113-127, fast_csqrt 155-175,
  *                  gk15_panel 180-187, adaptive_gk 193-212, gk_integrate 220-229
  *   bates_price    bates.rs:29-39, bates_call 49-63, bates_integrand 65-73, jump_cf 77-83
+ *   bates_jump_sens  ad.rs:882-895 (forward_pass 695-731 with the jump
+ *                  parameter active, jump_factor 218-234, dual_integrand
+ *                  609-630, adaptive_gk_dual 639-684): the price and its
+ *                  derivatives in lambda, mu_j, sigma_j, each derivative
+ *                  integrand (the exact derivative of the jump factor times
+ *                  the integrand) summed on the panels the value's error
+ *                  chooses
  *   bsm_price      bsm.rs:7-12, 55-60
  * plus the textbook Merton (1976) jump-diffusion series, which has no
  * reference counterpart and pins the Bates MC at zero vol-of-vol.
@@ -224,6 +231,121 @@ double ora_bates_price(double s, double k, double t, double r, double q, const o
     double call = cf_call(s, k, t, r, q, p, 1, lambda, mu_j, sigma_j);
     if (opt_type == ORA_CALL) return call;
     return call - s * exp(-q * t) + k * exp(-r * t);
+}
+
+/* ---------------------------------------------- jump-parameter derivatives */
+
+/* d ln J(phi) / d(lambda, mu_j, sigma_j) of jump_cf: lambda T (E - 1 - i phi k)
+ * with E = exp(i phi mu_j - sigma_j^2 phi^2 / 2), k = e^(mu_j + sigma_j^2/2) - 1,
+ * differentiated by hand (ad.rs:218-234 propagates the same with dual numbers). */
+static void jump_dlog(cpx phi, double t, double lambda, double mu_j, double sigma_j, cpx g[3]) {
+    const cpx I = C(0.0, 1.0);
+    double ec = exp(mu_j + 0.5 * sigma_j * sigma_j);
+    cpx E = cexp_(csub(cscale(cmul(phi, I), mu_j), cscale(cscale(cmul(cscale(phi, 0.5), phi), sigma_j), sigma_j)));
+    cpx iphi = cmul(I, phi);
+    cpx inner = csub(cradd(E, -1.0), cscale(iphi, ec - 1.0));
+    g[0] = cscale(inner, t);
+    g[1] = cscale(cmul(iphi, cradd(E, -ec)), lambda * t);
+    g[2] = cscale(cadd(cmul(cmul(phi, phi), E), cscale(iphi, ec)), -lambda * t * sigma_j);
+}
+
+/* integrand and its three jump derivatives on the t map; the value keeps the
+ * pricer's guard, a derivative is 0 where it or the value is not finite */
+static double mapped_d(const integrand_ctx *c, double t, double d[3]) {
+    d[0] = d[1] = d[2] = 0.0;
+    if (t <= 0.0) return 0.0;
+    double u = (1.0 - t) / t;
+    if (u < 1e-12) return 0.0;
+    cpx phi = c->is_p1 ? C(u, -1.0) : C(u, 0.0);
+    cpx cf = cmul(stable_cf(phi, c->t, c->r, c->p), jump_cf(phi, c->t, c->lambda, c->mu_j, c->sigma_j));
+    if (c->has_norm) cf = cdiv(cf, c->norm);
+    cpx z = cdiv(cmul(cexp_(C(0.0, u * c->x)), cf), C(0.0, u));
+    double v = z.re / (t * t);
+    if (!isfinite(v)) return 0.0;
+    cpx g[3];
+    jump_dlog(phi, c->t, c->lambda, c->mu_j, c->sigma_j, g);
+    for (int k = 0; k < 3; ++k) {
+        double dk = cmul(z, g[k]).re / (t * t);
+        d[k] = isfinite(dk) ? dk : 0.0;
+    }
+    return v;
+}
+
+static void gk15_panel_d(const integrand_ctx *c, double a, double b, double *kout, double *eout, double kd[3]) {
+    double cc = 0.5 * (a + b), h = 0.5 * (b - a);
+    double fv[15], fd[15][3];
+    for (int i = 0; i < 15; ++i) fv[i] = mapped_d(c, cc + h * GK_NODES[i], fd[i]);
+    double k = 0.0, g = 0.0;
+    for (int i = 0; i < 15; ++i) k += GK_WEIGHTS[i] * fv[i];
+    for (int j = 0; j < 7; ++j) g += G7_WEIGHTS[j] * fv[G7_IDX[j]];
+    for (int q = 0; q < 3; ++q) {
+        double s = 0.0;
+        for (int i = 0; i < 15; ++i) s += GK_WEIGHTS[i] * fd[i][q];
+        kd[q] = s * h;
+    }
+    *kout = k * h;
+    *eout = fabs(k - g) * h;
+}
+
+/* adaptive_gk carrying the three derivative sums on the same panels */
+static double adaptive_gk_d(const integrand_ctx *c, double tol, double dtot[3]) {
+    enum { MAX_PANELS = 200 };
+    double pe[MAX_PANELS + 2], pk[MAX_PANELS + 2], pa[MAX_PANELS + 2], pb[MAX_PANELS + 2];
+    double pd[MAX_PANELS + 2][3];
+    int n = 0;
+    double k0, e0, d0[3];
+    gk15_panel_d(c, 0.0, 1.0, &k0, &e0, d0);
+    pe[0] = e0; pk[0] = k0; pa[0] = 0.0; pb[0] = 1.0; n = 1;
+    for (int q = 0; q < 3; ++q) { pd[0][q] = d0[q]; dtot[q] = d0[q]; }
+    double total = k0, err = e0;
+    while (err > tol && n < MAX_PANELS) {
+        int w = 0;
+        for (int i = 1; i < n; ++i) if (pe[i] >= pe[w]) w = i;
+        double ew = pe[w], kw = pk[w], aw = pa[w], bw = pb[w], dw[3];
+        for (int q = 0; q < 3; ++q) { dw[q] = pd[w][q]; pd[w][q] = pd[n - 1][q]; }
+        pe[w] = pe[n - 1]; pk[w] = pk[n - 1]; pa[w] = pa[n - 1]; pb[w] = pb[n - 1];
+        n -= 1;
+        double m = 0.5 * (aw + bw);
+        double kl, el, kr, er, dl[3], dr[3];
+        gk15_panel_d(c, aw, m, &kl, &el, dl);
+        gk15_panel_d(c, m, bw, &kr, &er, dr);
+        total += kl + kr - kw;
+        err += el + er - ew;
+        for (int q = 0; q < 3; ++q) dtot[q] += dl[q] + dr[q] - dw[q];
+        pe[n] = el; pk[n] = kl; pa[n] = aw; pb[n] = m;
+        for (int q = 0; q < 3; ++q) pd[n][q] = dl[q];
+        n++;
+        pe[n] = er; pk[n] = kr; pa[n] = m; pb[n] = bw;
+        for (int q = 0; q < 3; ++q) pd[n][q] = dr[q];
+        n++;
+    }
+    return total;
+}
+
+/* ad.rs:882-895: out = {price, d/d lambda, d/d mu_j, d/d sigma_j} */
+void ora_bates_jump_sens(double s, double k, double t, double r, double q, const ora_heston *p, double lambda,
+                         double mu_j, double sigma_j, int opt_type, double *out) {
+    integrand_ctx c;
+    c.x = log(s / k);
+    c.t = t;
+    c.r = r;
+    c.p = p;
+    c.bates = 1;
+    c.lambda = lambda;
+    c.mu_j = mu_j;
+    c.sigma_j = sigma_j;
+    cpx mi = C(0.0, -1.0);
+    c.is_p1 = 1; c.has_norm = 1; c.norm = cmul(stable_cf(mi, t, r, p), jump_cf(mi, t, lambda, mu_j, sigma_j));
+    double d1[3], d2[3];
+    double i1 = adaptive_gk_d(&c, 1e-8, d1);
+    c.is_p1 = 0; c.has_norm = 0;
+    double i2 = adaptive_gk_d(&c, 1e-8, d2);
+    const double PI = 3.141592653589793;
+    double seq = s * exp(-q * t), ker = k * exp(-r * t);
+    double call = seq * (0.5 + i1 / PI) - ker * (0.5 + i2 / PI);
+    call = call > 0.0 ? call : 0.0;
+    out[0] = opt_type == ORA_CALL ? call : call - seq + ker;
+    for (int j = 0; j < 3; ++j) out[1 + j] = seq * (d1[j] / PI) - ker * (d2[j] / PI);
 }
 
 /* n options, 15 doubles each: spot, strike, expiry, rate, div_yield, v0,

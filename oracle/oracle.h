@@ -23,12 +23,13 @@
  *   ora_mc_price              mc.rs:148-207 (mc_heston 98-103 / mc_bates 107-113)
  *   ora_heston_price          heston.rs:61-229
  *   ora_bates_price           bates.rs:29-83
+ *   ora_bates_jump_sens       ad.rs:882-895 (price and d/d lambda, mu_j, sigma_j)
  *   ora_cf_price_many         a chain of the two above (batch.rs:23-42)
  *   ora_cf_greeks             greeks.rs:19-82 bump-and-reprice over the two above
  *   ora_bsm_greeks            bsm.rs:11-45 (and ora_bsm_greeks_many, a chain)
  *   ora_implied_vol           iv.rs:14-117 (and ora_implied_vol_many, a chain)
  *   ora_calib_residuals       calibration.rs:573-587 (calib.c)
- *   ora_calib_jacobian        calibration.rs:613-652, every column differenced (calib.c)
+ *   ora_calib_jacobian        calibration.rs:613-652, Bates jump columns by AD (calib.c)
  *   ora_calib_jtj_grad        calibration.rs:655-667 (calib.c)
  *   ora_bsm_price             bsm.rs:7-12, 55-60
  *   ora_merton_price          textbook Merton (1976) series (no reference counterpart)
