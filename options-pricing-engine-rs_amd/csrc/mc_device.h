@@ -76,6 +76,7 @@ __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrt
 __device__ __forceinline__ float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float hw_sin_rev(float x) { return __builtin_amdgcn_sinf(x); }  // sin(2 pi x)
 __device__ __forceinline__ float hw_cos_rev(float x) { return __builtin_amdgcn_cosf(x); }  // cos(2 pi x)
 __device__ __forceinline__ float clamp_pos(float x) {                                     // max(x, 0), one v_med3
@@ -403,38 +404,46 @@ __device__ __forceinline__ float ncdf_neg_abs_f32(float z) {
 }
 
 // QE variance step in FP32 (Andersen 2008, psi_c = 1.5) for one sibling, in
-// terms of the conditional mean m and half the conditional variance, s2h =
-// s2 / 2 (the host halves its coefficients), with no division by psi =
-// s2/m^2 = 2 s2h/m^2; psi > 1.5 <=> s2h > 0.75 m^2.  The two branches share
-// their reciprocals and the final product:
-// * quadratic: t = 2/psi = m^2/s2h (capped at 2e12, the psi floor 1e-12),
-//   1 + b^2 = t + sqrt(t (t - 1)), v' = m/(1 + b^2) (b + z)^2;
-// * exponential: t = 1 - p = m^2/(s2h + m^2/2), 1/beta = m/(1 - p), and with
-//   phic = 1 - ncdf(z): v' = 0 when u <= p (phic >= t), else
-//   ln(t/phic)/beta = m/t max(ln(t/phic), 0), which avoids the cancellation
-//   of 1 - u.
-// qe_sib_f32 returns m r and (b + z)^2 with r = 1/(1 + b^2) or 1/t; the
-// exponential lanes' (b + z)^2 is meaningless and is replaced by the caller.
-// With m = 0 (theta = 0 and v = 0) both m^2 and s2h are 0: t = 0 * inf is
-// NaN, which the cap turns into 2e12 (fminf returns its other operand), and
-// v' = m r (b + z)^2 = 0, as in the reference.
+// terms of the conditional mean m and u = psi/2 = s2h/m^2, s2h = s2/2 (the
+// host halves its coefficients); psi > 1.5 <=> u > 0.75.  u is floored at
+// the reference's psi floor (psi >= 1e-12) and capped at 1e30, so that a
+// vanishing m^2 leaves every product finite; with m = 0 (theta = 0 and v = 0)
+// u = 0 * inf is NaN, which the floor turns into 5e-13 (fmaxf returns its
+// other operand), and v' = 0 as in the reference.
+// * quadratic: with w = sqrt(1 - u), 2/psi = 1/u and
+//   b^2 = 2/psi - 1 + sqrt(2/psi (2/psi - 1)) = w (1 + w)/u,
+//   a = m/(1 + b^2) = m u/(1 + w), so a b^2 = m w and
+//   v' = a (b + z)^2 = m w (1 + z/b)^2 with 1/b = sqrt(u/(w (1 + w)))
+//   = u rsqrt(u w (1 + w)): three transcendentals, no division, and no
+//   cancellation for u in [0, 0.75];
+// * exponential: 1 - p = 2/(psi + 1) = 1/(u + 1/2) and 1/beta = m (u + 1/2),
+//   and with phic = 1 - ncdf(z): v' = 0 when u_z <= p (phic (u + 1/2) >= 1),
+//   else ln((1 - p)/phic)/beta = m (u + 1/2) max(-ln((u + 1/2) phic), 0),
+//   which avoids the cancellation of 1 - ncdf(z).
+// qe_sib_f32 returns m, u and the quadratic branch's v'; on exponential
+// lanes that v' is meaningless (sqrt of 1 - u < 0) and the caller replaces it.
 struct QeSib {
-    float mr, bz2, t;
+    float m, u, vq;
     bool ex;
 };
 
 __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z, float m0, float s0) {
     QeSib o;
-    const float m = fmaf(d.qe_m1, v, m0), s2h = fmaf(d.qe_s1, v, s0);
-    const float msq = m * m;
-    o.ex = s2h > 0.75f * msq;
-    o.t = fminf(msq * hw_rcp(o.ex ? fmaf(0.5f, msq, s2h) : s2h), 2e12f);
-    // 1 + b^2 = t + sqrt(t^2 - t), b^2 = (1 + b^2) - 1
-    const float opb = o.t + hw_sqrt(fmaf(o.t, o.t, -o.t));
-    const float bz = hw_sqrt(opb - 1.0f) + z;
-    o.bz2 = bz * bz;
-    o.mr = m * hw_rcp(o.ex ? o.t : opb);
+    o.m = fmaf(d.qe_m1, v, m0);
+    const float s2h = fmaf(d.qe_s1, v, s0);
+    o.u = fminf(fmaxf(s2h * hw_rcp(o.m * o.m), 5e-13f), 1e30f);
+    o.ex = o.u > 0.75f;
+    const float w = hw_sqrt(1.0f - o.u);
+    const float ib = o.u * hw_rsq(o.u * fmaf(w, w, w));  // 1/b
+    const float f = fmaf(z, ib, 1.0f);
+    o.vq = (o.m * w) * (f * f);
     return o;
+}
+
+// The exponential branch's v' for tail probability phic = 1 - ncdf(z).
+__device__ __forceinline__ float qe_exp_f32(const QeSib& s, float phic) {
+    const float uh = s.u + 0.5f;
+    return (s.m * uh) * fmaxf(hw_log2(uh * phic) * -0.69314718055994531f, 0.0f);
 }
 
 // One QE step of both siblings of a pair: sibling a draws (za, zb), sibling b
@@ -442,7 +451,7 @@ __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z, f
 // runtime flag costs register copies and selects at every step).  |za| is
 // common to both, so the exponential branch's erfc is evaluated once per
 // pair, and only when a lane of the wave needs it; its results replace the
-// quadratic branch's factors inside that branch, so that nothing has to be
+// quadratic branch's v' inside that branch, so that nothing has to be
 // initialised on the way round it.
 // The variances are >= 0 on entry (QE's v' is never negative, and v0 is
 // clamped by the caller).  The price variance term K3 v + K4 v' has K3 = K4
@@ -456,29 +465,15 @@ __device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& v
     const QeSib a = qe_sib_f32(d, va, za, m0, s0);
     QeSib b{};
     if (ANTI) b = qe_sib_f32(d, vb, -za, m0, s0);
-    // the factor of m r: (b + z)^2, or ln(t / (1 - ncdf(z))) on exponential lanes
-    float qa = a.bz2, qb = b.bz2;
+    float na = a.vq, nb = b.vq;
     if (a.ex || (ANTI && b.ex)) {
+        // z = za for sibling a, z = -za for sibling b: 1 - ncdf(z) is e =
+        // ncdf(-|za|) for the sibling whose z >= 0 and 1 - e for the other
         const float e = ncdf_neg_abs_f32(za), ome = 1.0f - e;
         const bool pos = za >= 0.0f;
-        // z = za for sibling a, z = -za for sibling b; with the sibling, one
-        // reciprocal serves both: 1/e = (1 - e) r and 1/(1 - e) = e r for
-        // r = 1/(e (1 - e)), e in (0, 1/2].  (One logarithm per lane, of the
-        // exponential sibling's argument, with a second pass for lanes where
-        // both siblings are exponential, measured 0.9 more transcendentals
-        // per pair-step: some lane of a wave almost always has both.)
-        if (ANTI) {
-            const float r = hw_rcp(e * ome);
-            const float la = fmaxf(hw_log2(a.t * ((pos ? ome : e) * r)), 0.0f) * 0.69314718055994531f;
-            const float lb = fmaxf(hw_log2(b.t * ((pos ? e : ome) * r)), 0.0f) * 0.69314718055994531f;
-            qa = a.ex ? la : qa;
-            qb = b.ex ? lb : qb;
-        } else {
-            const float la = fmaxf(hw_log2(a.t * hw_rcp(pos ? e : ome)), 0.0f) * 0.69314718055994531f;
-            qa = a.ex ? la : qa;
-        }
+        if (a.ex) na = qe_exp_f32(a, pos ? e : ome);
+        if (ANTI && b.ex) nb = qe_exp_f32(b, pos ? ome : e);
     }
-    const float na = a.mr * qa;
     const float zs = d.qe_sq3 * zb;
     auto incr = [&](float vp, float vn, float z) {
         float dy = fmaf(hw_sqrt(vp + vn), z, fmaf(d.qe_q2, vn, fmaf(d.qe_q1, vp, q0)));
@@ -488,7 +483,6 @@ __device__ __forceinline__ void qe_pair_f32(const McDesc& d, float& va, float& v
     dya = incr(va, na, zs);
     va = na;
     if (ANTI) {
-        const float nb = b.mr * qb;
         dyb = incr(vb, nb, -zs);
         vb = nb;
     } else {
