@@ -234,7 +234,7 @@ int ope_mc_batch(const ope_mc_request *reqs, uint32_t n, const ope_mc_options *o
  * a result does not depend on what else is being served.  A request that
  * stops after P pairs returns bit-for-bit what ope_mc_bates_ex returns for
  * n_paths = 2P (or P without antithetic pairs) when that call runs one pair
- * per thread (fewer than 2^20 pairs). */
+ * per thread (fewer than 2^22 pairs). */
 typedef struct ope_server ope_server;
 
 typedef struct ope_server_config {

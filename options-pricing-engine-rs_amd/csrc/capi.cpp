@@ -27,7 +27,11 @@ namespace ope {
 namespace host {
 
 constexpr double kLog2eD = 1.4426950408889634;
-constexpr uint64_t kPPT4Threshold = 1ull << 20;  // pairs at which a request uses 4 pairs per thread
+// Pairs at which a request uses 4 pairs per thread.  Below it one pair per
+// thread gives 4x the tiles, so the last round of tiles on the GPU is
+// fuller (the 8 x 2^21-pair Bates surface: 4.99 -> 4.94 ms with its strike
+// loop's cost per tile); above it the tile count is large either way.
+constexpr uint64_t kPPT4Threshold = 1ull << 22;
 constexpr uint64_t kMaxPairs = 1ull << 40;
 constexpr uint64_t kMaxSteps = 1ull << 24;
 constexpr double kMaxPoissonMean = 64.0;
@@ -274,6 +278,17 @@ int make_desc(const Request& r, McDesc& d) {
         const double thr = std::ceil(d.pois_e0 * 4294967296.0 - 0.5);
         d.pois_thr0 = thr >= 4294967295.0 ? 0xFFFFFFFFu : (thr <= 0.0 ? 0u : (uint32_t)thr);
         d.pois_gate = d.pois_thr0 >> 23;
+        // F1 = e0 + e0 mu / 1, the CDF after the device walk's first step,
+        // with that step's roundings (the division by 1 is exact); u < F1
+        // <=> w < ceil(F1 2^32 - 1/2), both sides exact.  Clamped below
+        // 2^32: the largest w then takes the walk, which gives the same count.
+        const double f1 = [&] {
+#pragma clang fp contract(off)
+            const double p1 = d.pois_e0 * pois_mu;
+            return d.pois_e0 + p1;
+        }();
+        const double thr1 = std::ceil(f1 * 4294967296.0 - 0.5);
+        d.pois_thr1 = thr1 >= 4294967295.0 ? 0xFFFFFFFFu : (thr1 <= 0.0 ? 0u : (uint32_t)thr1);
         d.mu_j = r.mu_j;
         d.sigma_j = r.sigma_j;
         d.f_mu_j2 = (float)(kLog2eD * r.mu_j);

@@ -52,7 +52,7 @@ int with_context(int (*body)(void* arg, const ContextView& view), void* arg);
 int check_options(const ope_mc_options* o);
 
 // A single request's tiling as every entry point plans it: pairs per thread
-// (4 from 2^20 pairs, else 1), tiles of kBlock * ppt pairs, one strike.
+// (4 from 2^22 pairs, else 1), tiles of kBlock * ppt pairs, one strike.
 // Returns OPE_OK or OPE_ERR_INVALID_ARGUMENT for too many tiles.
 int set_single_tiling(McDesc& d);
 

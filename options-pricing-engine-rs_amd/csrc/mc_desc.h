@@ -37,6 +37,7 @@ struct alignas(16) McDesc {
     uint32_t out_off;     // into the (sum, sum_sq) result array
     uint32_t pois_thr0;   // w < thr0  <=>  (w + 0.5) 2^-32 < e^-mu  (Poisson count 0)
     uint32_t pois_gate;   // thr0 >> 23: 9 high bits below it mean count 0 whatever the low bits
+    uint32_t pois_thr1;   // thr0 <= w < thr1  =>  count 1 (the CDF walk's first step, done on the host)
     uint32_t flags;       // OPE_FLAG_* of the call (bit 0: FP64 barrier decision near B)
 
     // ---- FP64 reference-arithmetic constants

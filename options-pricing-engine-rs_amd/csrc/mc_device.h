@@ -174,7 +174,7 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
     const uint4 w = st.call(s, kTagJump);
     const uint32_t W = (h << (32u - kGateBits)) | (w.z >> kGateBits);
     if (W >= d.pois_thr0) {
-        n = poisson_tail(W, d.pois_mu, d.pois_e0);
+        n = W < d.pois_thr1 ? 1u : poisson_tail(W, d.pois_mu, d.pois_e0);
         if constexpr (FP32) {
             float z, unused;
             box_muller_f32(w.x, w.y, z, unused);
@@ -204,7 +204,8 @@ struct NoBlockEnd {
 // of the slots are drawn after the last step and added to *jsum.  A wave then
 // runs the rare jump code about three times per path instead of at every
 // step where one of its lanes may jump.  A lane's fourth and later events
-// are drawn at once.  The draws are the same words either way.
+// are drawn at once.  The draws are the same words either way.  (The slots
+// measured faster in registers than as one LDS store per event.)
 //
 // block_end(s) is called after step s for every s = 8k + 7 of the steps
 // that complete a block of eight (FP32 mode, in which the step sums its
@@ -224,6 +225,8 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     using R = typename std::conditional<FP32, float, double>::type;
     static_assert(!DEFER || JUMPS, "deferred jumps need the jump model");
     static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
+    // DEFER: the lane's recorded events, oldest first, as (step, gate bits);
+    // step ~0u marks an empty slot
     uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u, es2 = ~0u, ew2 = 0u;
     R jd = 0;
     const float phi = MIX ? in_vgpr(d.eu_phi) : 0.0f;
@@ -942,6 +945,18 @@ struct PathSink {
     double* state;   // [pair][2][4]
 };
 
+// LDS of a tile block (double2 words): the FP64 strike loop keeps one
+// (sum, sum_sq) per (strike, wave); the FP32 strike loop transposes the
+// lane sums of kEpiStrikes strikes through rows of kEpiRow float2 (one per
+// thread, padded by one against bank conflicts), then keeps one FP64 pair
+// per (strike of the chunk, wave) at kEpiWsumOff.
+constexpr int kEpiStrikes = 8;
+constexpr int kEpiRow = kBlock + 1;
+constexpr int kEpiWsumOff = (kEpiStrikes * kEpiRow * 8 + 15) / 16;
+constexpr int kTileLds = kMaxStrikes * 4 > kEpiWsumOff + kEpiStrikes * 4 ? kMaxStrikes * 4
+                                                                          : kEpiWsumOff + kEpiStrikes * 4;
+static_assert(kBlock == 256 && kEpiStrikes == 8, "thread t sums threads 8 (t >> 3) .. +7 of row t & 7");
+
 // One tile: kBlock * PPT pairs of one request.  Thread t simulates pairs
 // tile_base + j*kBlock + t (j < PPT), then for every strike the block sums
 // the discounted payoffs and their squares in a fixed order (per-thread j
@@ -1001,32 +1016,99 @@ __device__ __forceinline__ void price_tile(const McDesc& d, uint32_t tile, const
     // the lane's payoffs are summed undiscounted and the sums discounted once
     // (disc and disc^2): sum and sum of squares of disc * payoff up to rounding
     const double sgn = d.opt_put ? -1.0 : 1.0, disc2 = d.disc * d.disc;
-    for (uint32_t k = 0; k < ns; ++k) {
-        const double nk = -sgn * strikes[d.strike_off + k];
-        double s = 0.0, q = 0.0;
+    if constexpr (sizeof(Real) == 4) {
+        // FP32 path: each lane's own payoffs (at most 2 PPT) and their sums
+        // in FP32; every sum over lanes, waves and tiles in FP64.  A path that
+        // does not exist or is knocked out enters as sgn x = -inf, whose
+        // intrinsic value is 0; the knocked-out ones add the rebate instead,
+        // the same for every strike.
+        constexpr int M = 2 * PPT;
+        float sx[M];
+        float nko = 0.0f;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-            if (valid[j]) {
-                const double pa = payoff_undiscounted(d, ea[j], sgn, nk);
-                s += pa;
-                q += pa * pa;
-                if (anti) {
-                    const double pb = payoff_undiscounted(d, eb[j], sgn, nk);
-                    s += pb;
-                    q += pb * pb;
+            const bool oka = valid[j], okb = valid[j] && anti;
+            sx[2 * j] = oka && !ea[j].ko ? (float)(sgn * ea[j].x) : -__builtin_huge_valf();
+            sx[2 * j + 1] = okb && !eb[j].ko ? (float)(sgn * eb[j].x) : -__builtin_huge_valf();
+            nko += (oka && ea[j].ko ? 1.0f : 0.0f) + (okb && eb[j].ko ? 1.0f : 0.0f);
+        }
+        const float cs = nko * (float)d.rebate, cq = cs * (float)d.rebate;
+        // Strikes in chunks of kEpiStrikes: the block's lane sums go through
+        // LDS transposed (row c of the chunk holds every thread's sums for
+        // strike c), thread t then adds the 8 threads 8 (t >> 3) .. +7 of row
+        // t & 7 in FP64, the xor butterfly over the wave's 8 groups and the
+        // 4 waves in order complete the tile's sum, and the sums are
+        // discounted once.  The order depends on the request alone.
+        float2* rows = reinterpret_cast<float2*>(lds);
+        double2* wsum = lds + kEpiWsumOff;
+        const int c = tid & (kEpiStrikes - 1), g = tid >> 3;
+        for (uint32_t k0 = 0; k0 < ns; k0 += kEpiStrikes) {
+#pragma unroll
+            for (int cc = 0; cc < kEpiStrikes; ++cc) {
+                float s = 0.0f, q = 0.0f;
+                if (k0 + cc < ns) {
+                    const float nk = (float)(-sgn * strikes[d.strike_off + k0 + cc]);
+                    s = cs;
+                    q = cq;
+#pragma unroll
+                    for (int i = 0; i < M; ++i) {
+                        const float p = fmaxf(sx[i] + nk, 0.0f);
+                        s += p;
+                        q = fmaf(p, p, q);
+                    }
                 }
+                rows[cc * kEpiRow + tid] = make_float2(s, q);
+            }
+            __syncthreads();
+            double s = 0.0, q = 0.0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float2 v = rows[c * kEpiRow + g * 8 + i];
+                s += (double)v.x;
+                q += (double)v.y;
+            }
+#pragma unroll
+            for (int m = 8; m < 64; m <<= 1) {
+                s += __shfl_xor(s, m, 64);
+                q += __shfl_xor(q, m, 64);
+            }
+            if (lane < kEpiStrikes) wsum[lane * 4 + wave] = make_double2(s, q);
+            __syncthreads();
+            if (tid < kEpiStrikes && k0 + tid < ns) {
+                const double2 a = wsum[tid * 4 + 0], b = wsum[tid * 4 + 1], e = wsum[tid * 4 + 2],
+                              f = wsum[tid * 4 + 3];
+                slots[d.slot_off + (uint64_t)tile * ns + k0 + tid] =
+                    make_double2((((a.x + b.x) + e.x) + f.x) * d.disc, (((a.y + b.y) + e.y) + f.y) * disc2);
             }
         }
-        s *= d.disc;
-        q *= disc2;
-        wave_sum2(s, q);
-        if (lane == 0) lds[k * 4 + wave] = make_double2(s, q);
-    }
-    __syncthreads();
-    for (uint32_t k = tid; k < ns; k += kBlock) {
-        const double2 a = lds[k * 4 + 0], b = lds[k * 4 + 1], c = lds[k * 4 + 2], e = lds[k * 4 + 3];
-        slots[d.slot_off + (uint64_t)tile * ns + k] =
-            make_double2(((a.x + b.x) + c.x) + e.x, ((a.y + b.y) + c.y) + e.y);
+    } else {
+        for (uint32_t k = 0; k < ns; ++k) {
+            const double nk = -sgn * strikes[d.strike_off + k];
+            double s = 0.0, q = 0.0;
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                if (valid[j]) {
+                    const double pa = payoff_undiscounted(d, ea[j], sgn, nk);
+                    s += pa;
+                    q += pa * pa;
+                    if (anti) {
+                        const double pb = payoff_undiscounted(d, eb[j], sgn, nk);
+                        s += pb;
+                        q += pb * pb;
+                    }
+                }
+            }
+            s *= d.disc;
+            q *= disc2;
+            wave_sum2(s, q);
+            if (lane == 0) lds[k * 4 + wave] = make_double2(s, q);
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < ns; k += kBlock) {
+            const double2 a = lds[k * 4 + 0], b = lds[k * 4 + 1], c = lds[k * 4 + 2], e = lds[k * 4 + 3];
+            slots[d.slot_off + (uint64_t)tile * ns + k] =
+                make_double2(((a.x + b.x) + c.x) + e.x, ((a.y + b.y) + c.y) + e.y);
+        }
     }
 }
 

@@ -16,7 +16,7 @@ __global__ __launch_bounds__(kBlock, (pinned_waves<Real, CODE>())) void k_tiles_
                                                         const uint2* __restrict__ tiles,
                                                         const double* __restrict__ strikes,
                                                         double2* __restrict__ slots) {
-    __shared__ double2 lds[kMaxStrikes * 4];
+    __shared__ double2 lds[kTileLds];
     const uint2 t = tiles[blockIdx.x];
     run_code<Real, CODE>(descs[t.x], t.y, strikes, slots, lds);
 }
@@ -27,7 +27,7 @@ __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinned
                                                         const uint2* __restrict__ tiles,
                                                         const double* __restrict__ strikes,
                                                         double2* __restrict__ slots) {
-    __shared__ double2 lds[kMaxStrikes * 4];
+    __shared__ double2 lds[kTileLds];
     const uint2 t = tiles[blockIdx.x];
     const McDesc& d = descs[t.x];
     dispatch<Real, 0>(config_code(d), d, t.y, strikes, slots, lds);

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kBlock) void k_replay(const McDesc* __restrict__ de
     constexpr int kPayoff = (CODE >> 1) % 3;
     constexpr bool kJumps = (((CODE >> 1) / 3) & 1) != 0;
     constexpr int kScheme = ((CODE >> 1) / 3) >> 1;
-    __shared__ double2 lds[kMaxStrikes * 4];
+    __shared__ double2 lds[kTileLds];
     price_tile<Real, kScheme, kJumps, kPayoff, 1>(*desc, blockIdx.x, strikes, slots, ReplayMaker{za, zb, jump},
                                                   PathSink{pay_out, state_out}, lds);
 }

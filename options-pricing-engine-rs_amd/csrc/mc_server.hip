@@ -91,7 +91,7 @@ __device__ void sentinel(const SrvLaunch& L, SrvMirror* mirror) {
 
 template <class Real>
 __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinnedWavesServerF32 : 1)) void k_server(SrvLaunch L, SrvMirror* mirror) {
-    __shared__ double2 lds[kMaxStrikes * 4];
+    __shared__ double2 lds[kTileLds];
     __shared__ uint32_t s_rec[kRecWords];
     __shared__ SrvItem s_item;
     __shared__ uint32_t s_status;  // 0 work, 1 leave

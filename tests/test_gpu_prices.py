@@ -197,11 +197,11 @@ def test_reference_api_shapes(ope):
 def test_long_paths_and_ragged_counts(ope, scheme):
     # 4096 steps (dt = 2.4e-4): the draw loops' blocks of 8 steps, the
     # 8-step FP32 partial sums and the FP64 fold over a long path; an odd path
-    # count just above the 4-pairs-per-thread threshold (2^20 pairs) drops
+    # count just above the 4-pairs-per-thread threshold (2^22 pairs) drops
     # its last path (mc.rs:160-161) and ends in a ragged tile.  FP32 and FP64
     # see the same draws, so they agree to a small fraction of the SE.
     hp = ope.HestonParams(**BASE)
-    for n_paths, n_steps in ((20_001, 4096), (2 * (1 << 20) + 2 * 37 + 1, 13)):
+    for n_paths, n_steps in ((20_001, 4096), (2 * (1 << 22) + 2 * 37 + 1, 13)):
         cfg = ope.McConfig(n_paths=n_paths, n_steps=n_steps, seed=77, scheme=scheme)
         for pay in (ope.Payoff.European(100.0), ope.Payoff.AsianArithmetic(100.0)):
             r64 = run(ope, 0, pay, cfg, hp=BASE, jumps=(0.5, -0.1, 0.15))
