@@ -807,7 +807,7 @@ int ope_debug_philox(uint64_t seed, const uint32_t* ctr, uint32_t n, uint32_t* o
 
 int ope_debug_fp64_math(int32_t fn, const double* x, uint32_t n, double* out) {
     if ((!x || !out) && n) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
-    if (fn < 0 || fn > 4) return fail(OPE_ERR_INVALID_ARGUMENT, "unknown function %d", fn);
+    if (fn < 0 || fn > 5) return fail(OPE_ERR_INVALID_ARGUMENT, "unknown function %d", fn);
     if (fn == 2 || fn == 3)
         for (uint32_t i = 0; i < n; ++i)
             if (!(x[i] >= 0.0 && x[i] < 8388608.0 && x[i] == std::floor(x[i])))

@@ -7,6 +7,8 @@
 //
 // Methods:
 //   rcp_f64     v_rcp_f64 refined by two Newton-Raphson steps, y += y (1 - x y).
+//   rsqrt_f64   v_rsq_f64 refined by two Newton-Raphson steps,
+//               y += (y/2)(1 - x y^2).
 //   log_f64     x = 2^k m with m in [sqrt(1/2), sqrt 2) (frexp, exact);
 //               f = m - 1 exactly (Sterbenz); with s = f / (2 + f),
 //               log(1 + f) = f - (f^2/2 - s (f^2/2 + R)), R = sum 2 s^2j/(2j+1)
@@ -37,6 +39,15 @@ __device__ __forceinline__ double rcp_f64(double x) {
     y = fma(y, e, y);
     e = fma(-x, y, 1.0);
     return fma(y, e, y);
+}
+
+// 1/sqrt(x) for a positive normal x.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
 }
 
 // Natural logarithm of a positive normal x.

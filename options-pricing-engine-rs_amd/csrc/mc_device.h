@@ -565,25 +565,31 @@ __device__ __forceinline__ double euler_f64_fast(const McDesc& d, double& v, dou
     return inc;
 }
 
-// The QE variance step of one sibling in FP64, as qe_sib_f32: m, half the
-// conditional variance s2h, t = 2/psi (quadratic branch) or 1 - p
-// (exponential branch), and the factors m r and (b + z)^2.
+// The QE variance step of one sibling in FP64, as qe_sib_f32: m, u = psi/2
+// (floored at the psi floor 1e-12, capped at 1e30) and the quadratic
+// branch's v' = m w (1 + z/b)^2, w = sqrt(1 - u), 1/b = u rsqrt(u w (1 + w)).
 struct QeSib64 {
-    double mr, bz2, t;
+    double m, u, vq;
     bool ex;
 };
 
 __device__ __forceinline__ QeSib64 qe_sib_f64(const McDesc& d, double v, double z) {
     QeSib64 o;
-    const double m = fma(d.ekt, v, d.qd_m0), s2h = fma(d.qd_s1, v, d.qd_s0);
-    const double msq = m * m;
-    o.ex = s2h > 0.75 * msq;
-    o.t = fmin(msq * rcp_f64(o.ex ? fma(0.5, msq, s2h) : s2h), 2e12);  // psi floor 1e-12
-    const double opb = o.t + sqrt(fma(o.t, o.t, -o.t));                  // 1 + b^2
-    const double bz = sqrt(opb - 1.0) + z;
-    o.bz2 = bz * bz;
-    o.mr = m * rcp_f64(o.ex ? o.t : opb);
+    o.m = fma(d.ekt, v, d.qd_m0);
+    const double s2h = fma(d.qd_s1, v, d.qd_s0);
+    o.u = fmin(fmax(s2h * rcp_f64(o.m * o.m), 5e-13), 1e30);
+    o.ex = o.u > 0.75;
+    const double w = sqrt(1.0 - o.u);
+    const double ib = o.u * rsqrt_f64(o.u * fma(w, w, w));
+    const double f = fma(z, ib, 1.0);
+    o.vq = (o.m * w) * (f * f);
     return o;
+}
+
+// The exponential branch's v' for tail probability phic = 1 - ncdf(z).
+__device__ __forceinline__ double qe_exp_f64(const QeSib64& s, double phic) {
+    const double uh = s.u + 0.5;
+    return (s.m * uh) * fmax(-log_f64(uh * phic), 0.0);
 }
 
 // One QE step of both siblings (mc.rs:256-267, 288-309): returns the ln-price
@@ -596,22 +602,19 @@ __device__ __forceinline__ void qe_pair_f64_fast(const McDesc& d, bool anti, dou
     const QeSib64 a = qe_sib_f64(d, va, za);
     QeSib64 b{};
     if (anti) b = qe_sib_f64(d, vb, -za);
-    double qa = a.bz2, qb = b.bz2;
+    double na = a.vq, nb = b.vq;
     if (a.ex || (anti && b.ex)) {
         // e = ncdf(-|za|) (math.rs:16-19); 1 - ncdf(z) is e for the sibling
-        // whose z >= 0 and 1 - e for the other; one reciprocal serves both
+        // whose z >= 0 and 1 - e for the other
         const double e = 0.5 * erfc(fabs(za) * 0.7071067811865476), ome = 1.0 - e;
         const bool pos = za >= 0.0;
-        const double r = rcp_f64(e * ome);
-        qa = a.ex ? fmax(log_f64(a.t * ((pos ? ome : e) * r)), 0.0) : qa;
-        if (anti) qb = b.ex ? fmax(log_f64(b.t * ((pos ? e : ome) * r)), 0.0) : qb;
+        if (a.ex) na = qe_exp_f64(a, pos ? e : ome);
+        if (anti && b.ex) nb = qe_exp_f64(b, pos ? ome : e);
     }
     const double zs = d.qd_sq3 * zb;
-    const double na = a.mr * qa;
     dya = fma(sqrt(va + na), zs, fma(d.k2, na, fma(d.k1, va, d.qe_a0)));
     va = na;
     if (anti) {
-        const double nb = b.mr * qb;
         dyb = fma(sqrt(vb + nb), -zs, fma(d.k2, nb, fma(d.k1, vb, d.qe_a0)));
         vb = nb;
     } else {

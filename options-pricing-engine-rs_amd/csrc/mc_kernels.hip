@@ -98,8 +98,10 @@ __global__ void k_debug_fp64_math(int fn, const double* __restrict__ x, uint32_t
         double sn, cs;
         sincos_rev23((uint32_t)v, sn, cs);
         r = fn == 2 ? sn : cs;
-    } else {
+    } else if (fn == 4) {
         r = rcp_f64(v);
+    } else {
+        r = rsqrt_f64(v);
     }
     out[i] = r;
 }

@@ -434,7 +434,8 @@ def debug_philox(seed: int, ctr: np.ndarray) -> np.ndarray:
 
 def debug_fp64_math(fn: int, x) -> np.ndarray:
     """The FP64 path's elementary functions on the GPU: fn 0 log, 1 exp,
-    2 / 3 sin / cos of 2 pi x 2^-23 (integer x), 4 reciprocal."""
+    2 / 3 sin / cos of 2 pi x 2^-23 (integer x), 4 reciprocal, 5 reciprocal
+    square root."""
     x = np.ascontiguousarray(x, dtype=np.float64).ravel()
     out = np.zeros_like(x)
     P = C.POINTER(C.c_double)

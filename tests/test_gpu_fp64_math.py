@@ -83,6 +83,16 @@ def test_reciprocal(ope):
     assert err.max() <= 1.0
 
 
+def test_reciprocal_square_root(ope):
+    rng = np.random.default_rng(5)
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 3000), rng.uniform(0.2, 3.0, 2000),
+                        np.array([1.0, 4.0, 0.25, 2.0])])
+    got = ope.debug_fp64_math(5, x)
+    err = ulp_err(got, [1 / mpmath.sqrt(mpmath.mpf(v)) for v in x])
+    print(f"rsqrt: max {err.max():.2f} ulp", flush=True)
+    assert err.max() <= 1.0
+
+
 def test_angle_must_be_an_integer(ope):
     with pytest.raises(ope.OpeError):
         ope.debug_fp64_math(2, np.array([0.5]))
