@@ -248,6 +248,12 @@ int make_desc(const Request& r, McDesc& d) {
         d.qd_s1 = 0.5 * (hp.sigma * hp.sigma * d.ekt * d.one_m_ekt / hp.kappa);
         d.qd_s0 = 0.5 * d.qe_c2;
         d.qd_sq3 = std::sqrt(std::fmax(d.k3, 0.0));
+        // regrouping threshold: u(v) = (s1 v + s0) / (ekt v + m0)^2 = c, c = 0.1,
+        // i.e. c ekt^2 v^2 + (2 c ekt m0 - s1) v + c m0^2 - s0 = 0; u falls
+        // with v, so v below the positive root has u > c (none if u(0) <= c)
+        const double c = 0.1, qa = c * d.ekt * d.ekt, qb = 2.0 * c * d.ekt * d.qd_m0 - d.qd_s1,
+                     qc = c * d.qd_m0 * d.qd_m0 - d.qd_s0;
+        d.qe_vlow = qc < 0.0 ? (float)((-qb + std::sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa)) : 0.0f;
     }
     d.ed_a0 = drift * dt;
     d.ed_a1 = -0.5 * dt;

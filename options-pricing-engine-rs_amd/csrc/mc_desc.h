@@ -67,7 +67,7 @@ struct alignas(16) McDesc {
     float qe_s1, qe_s0;            // s2 / 2 = s1 v+ + s0 (halved on the host)
     float qe_q0, qe_q1, qe_q2;     // dy = q0 + q1 v+ + q2 v' + sq3 sqrt(v+ + v') zb
     float qe_sq3;                  // sqrt(max(q3, 0)), q3 = q4 the log2-scaled K3 = K4
-    float pad_f_;
+    float qe_vlow;                 // QE regrouping: v below it has u = psi/2 > 0.1 (0: no such v)
     float f_mu_j2, f_sigma_j2;     // log2e * mu_j, log2e * sigma_j
 };
 

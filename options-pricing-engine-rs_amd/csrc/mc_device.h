@@ -189,7 +189,7 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
 }
 
 struct NoBlockEnd {
-    __device__ void operator()(uint32_t) const {}
+    __device__ void operator()(uint32_t, PairStream&) const {}
 };
 
 // Drives the Philox-mode draws of one pair through all steps, calling
@@ -207,11 +207,13 @@ struct NoBlockEnd {
 // are drawn at once.  The draws are the same words either way.  (The slots
 // measured faster in registers than as one LDS store per event.)
 //
-// block_end(s) is called after step s for every s = 8k + 7 of the steps
+// block_end(s, st) is called after step s for every s = 8k + 7 of the steps
 // that complete a block of eight (FP32 mode, in which the step sums its
-// increments in FP32 over such blocks); the loops run whole blocks, so the
-// call sits at the end of a loop body rather than behind a per-step branch,
-// whose two sides would need register copies.
+// increments in FP32 over such blocks, and FP64 mode when a block_end is
+// given); the loops run whole blocks, so the call sits at the end of a loop
+// body rather than behind a per-step branch, whose two sides would need
+// register copies.  block_end may hand the lane another pair's state and
+// stream (st, the pair index words; see the QE regrouping).
 //
 // MIX (FP32 Euler): step() gets the scaled, correlated normals of the Euler
 // step, sa = a2 za and sb = b2 (rho za + sqrt(1 - rho^2) zb), instead of
@@ -219,12 +221,14 @@ struct NoBlockEnd {
 // sqrt(1 - rho^2) = sin phi, the mixed normal is r cos(t - phi): one cosine
 // replaces the sine and the mixing products, and a2, b2 fold into r.
 template <bool FP32, bool JUMPS, bool DEFER = false, bool MIX = false, class Step, class BlockEnd = NoBlockEnd>
-__device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& st, Step&& step,
+__device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& st0, Step&& step,
                                              typename std::conditional<FP32, float, double>::type* jsum = nullptr,
                                              BlockEnd block_end = BlockEnd{}) {
     using R = typename std::conditional<FP32, float, double>::type;
     static_assert(!DEFER || JUMPS, "deferred jumps need the jump model");
     static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
+    constexpr bool kBlocks = FP32 || !std::is_same<BlockEnd, NoBlockEnd>::value;
+    PairStream st = st0;
     // DEFER: the lane's recorded events, oldest first, as (step, gate bits);
     // step ~0u marks an empty slot
     uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u, es2 = ~0u, ew2 = 0u;
@@ -272,7 +276,7 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     };
     const uint32_t N = d.n_steps;
     uint32_t i = 0;
-    if constexpr (FP32) {
+    if constexpr (kBlocks) {
         // blocks of eight steps; FP32 Euler unrolls a block (the scheduler
         // overlaps consecutive steps), the QE step, three times longer,
         // measured slower unrolled and keeps a two-step body
@@ -292,7 +296,7 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
                     one(w.z, w.w, i + k + 1);
                 }
             }
-            block_end(i + 7);
+            block_end(i + 7, st);
         }
     }
     for (; i + 2 <= N; i += 2) {
@@ -311,6 +315,74 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
         if (es2 != ~0u) jd += jump_of_step<FP32>(d, st, es2, ew2, nn);
         *jsum += jd;
     }
+}
+
+// ------------------------------------ QE: regrouping pairs by variance
+// QE's exponential branch serves the pairs whose variance is close to 0 (psi
+// = s^2/m^2 > 1.5), a few in a hundred at the parameters that matter; but
+// one such lane makes its whole wave run the branch, which happens in ~90 %
+// of wave-steps when pairs stay on their lanes: a fifth of the FP32 QE
+// kernel's time, two fifths of the FP64 one's.  Low variance persists over
+// steps, so every 8 steps (at the block ends of the step driver) the tile
+// block regroups its pairs: the lanes whose variance is below the host's
+// threshold (u = psi/2 > 0.1 there; mc_desc.h qe_vlow) move to the front
+// rows in wave and lane order, the others follow, the order rotated by 64
+// rows per block and per regrouping so that no SIMD always holds the low
+// wave.  A lane hands over its whole pair (state words and Philox pair
+// index) through LDS; after the last step every pair returns to its home
+// lane.  A pair's steps are the same instructions on the same values
+// wherever it runs, so results do not change; waves with no low lane skip
+// the branch (57 % of wave-steps run it instead of 90 %, in a CPU model of
+// the variance paths at v0 = theta = 0.04, kappa = 1.5, sigma = 0.5, 365
+// steps; measured: FP32 QE pair 1.08 -> 1.04 ms, FP64 5.32 -> 5.07 ms, of
+// which the regrouping itself costs ~3 %).  Every thread of the block must
+// step (two __syncthreads per regrouping).
+constexpr int kGroupQuads = 4;  // most 16-byte quads of state a lane hands over (FP64 Asian / up-and-out)
+struct GroupLds {
+    uint4 rows[kGroupQuads][kBlock];
+    uint32_t cnt[kBlock / 64];
+};
+
+// Row this lane's pair moves to: low lanes first (wave order, then lane
+// order), then the rest, rotated by 64 rot rows.  One __syncthreads.
+__device__ __forceinline__ uint32_t group_dest(GroupLds* g, bool low, uint32_t rot) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint64_t m = __ballot(low);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (lane == 0) g->cnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    const uint32_t c0 = g->cnt[0], c1 = g->cnt[1], c2 = g->cnt[2], c3 = g->cnt[3];
+    const uint32_t pre = (wave > 0 ? c0 : 0u) + (wave > 1 ? c1 : 0u) + (wave > 2 ? c2 : 0u);
+    const uint32_t dst = low ? pre + rank : (c0 + c1 + c2 + c3) + (64u * wave - pre) + (lane - rank);
+    return (dst + 64u * rot) & (uint32_t)(kBlock - 1);
+}
+
+// Writes rec (W words, in 16-byte quads) to row dst and takes this thread's
+// row.  One __syncthreads; the rows are free again once every thread has
+// passed the next barrier.
+template <int W>
+__device__ __forceinline__ void group_move(GroupLds* g, uint32_t dst, uint32_t (&rec)[W]) {
+    static_assert(W % 4 == 0 && W / 4 <= kGroupQuads, "state quads");
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) g->rows[q][dst] = make_uint4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+        const uint4 v = g->rows[q][threadIdx.x];
+        rec[4 * q] = v.x;
+        rec[4 * q + 1] = v.y;
+        rec[4 * q + 2] = v.z;
+        rec[4 * q + 3] = v.w;
+    }
+}
+
+__device__ __forceinline__ void put_f64(uint32_t* r, double x) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    r[0] = (uint32_t)b;
+    r[1] = (uint32_t)(b >> 32);
+}
+__device__ __forceinline__ double get_f64(const uint32_t* r) {
+    return __longlong_as_double((long long)(((uint64_t)r[1] << 32) | r[0]));
 }
 
 // -------------------------------------------------- FP64 path (reference)
@@ -622,10 +694,11 @@ __device__ __forceinline__ void qe_pair_f64_fast(const McDesc& d, bool anti, dou
     }
 }
 
-// Pair simulation, FP64, Philox mode.
-template <int SCHEME, bool JUMPS, int PAYOFF>
+// Pair simulation, FP64, Philox mode.  GROUP: the block regroups its pairs
+// by variance every 8 steps (as the FP32 QE loop).
+template <int SCHEME, bool JUMPS, int PAYOFF, bool GROUP>
 __device__ __forceinline__ void sim_pair_f64_fast(const McDesc& d, const PhiloxSource& src, bool anti,
-                                                  PathState& ea, PathState& eb) {
+                                                  PathState& ea, PathState& eb, void* lds) {
     double la = d.ln_spot, lb = d.ln_spot;
     double va = fmax(d.v0, 0.0), vb = va;
     double suma = 0.0, sumb = 0.0, jsum = 0.0;
@@ -660,7 +733,54 @@ __device__ __forceinline__ void sim_pair_f64_fast(const McDesc& d, const PhiloxS
             mxb = fmax(mxb, lb);
         }
     };
-    philox_steps<false, JUMPS, kDefer>(d, src.st, step, &jsum);
+    if constexpr (GROUP) {
+        static_assert(SCHEME == kSchemeQE && !kDefer, "regrouping: QE without deferred jumps");
+        GroupLds* g = static_cast<GroupLds*>(lds);
+        uint32_t home = threadIdx.x;
+        constexpr int W = 12 + (PAYOFF == kPayAsian || PAYOFF == kPayUpAndOut ? 4 : 0);
+        auto move = [&](uint32_t dst, PairStream& st) {
+            uint32_t r[W] = {};
+            put_f64(r + 0, la);
+            put_f64(r + 2, lb);
+            put_f64(r + 4, va);
+            put_f64(r + 6, vb);
+            r[8] = st.plo;
+            r[9] = home;
+            if constexpr (PAYOFF == kPayAsian) {
+                put_f64(r + 10, suma);
+                put_f64(r + 12, sumb);
+            }
+            if constexpr (PAYOFF == kPayUpAndOut) {
+                put_f64(r + 10, mxa);
+                put_f64(r + 12, mxb);
+            }
+            group_move(g, dst, r);
+            la = get_f64(r + 0);
+            lb = get_f64(r + 2);
+            va = get_f64(r + 4);
+            vb = get_f64(r + 6);
+            st.plo = r[8];
+            home = r[9];
+            if constexpr (PAYOFF == kPayAsian) {
+                suma = get_f64(r + 10);
+                sumb = get_f64(r + 12);
+            }
+            if constexpr (PAYOFF == kPayUpAndOut) {
+                mxa = get_f64(r + 10);
+                mxb = get_f64(r + 12);
+            }
+        };
+        auto block_end = [&](uint32_t s, PairStream& st) {
+            const double vk = anti ? fmin(va, vb) : va;
+            move(group_dest(g, vk < (double)d.qe_vlow, (blockIdx.x + (s >> 3)) & 3u), st);
+        };
+        philox_steps<false, JUMPS, kDefer>(d, src.st, step, &jsum, block_end);
+        PairStream st = src.st;  // every pair back to its home lane
+        __syncthreads();
+        move(home, st);
+    } else {
+        philox_steps<false, JUMPS, kDefer>(d, src.st, step, &jsum);
+    }
     if (kDefer) {
         la += jsum;
         lb += jsum;
@@ -692,11 +812,12 @@ __device__ __forceinline__ void sim_pair_f64_fast(const McDesc& d, const PhiloxS
 // --------------------------------------------- pair simulation, FP64 path
 // Replay mode runs the reference's expressions in the reference's order
 // (mc.rs:243-277); Philox mode the hoisted form above.
-template <int SCHEME, bool JUMPS, int PAYOFF, class Src>
+// GROUP: as sim_pair_f64_fast (lds: the block's LDS).
+template <int SCHEME, bool JUMPS, int PAYOFF, bool GROUP = false, class Src>
 __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bool anti, PathState& ea,
-                                             PathState& eb) {
+                                             PathState& eb, void* lds = nullptr) {
     if constexpr (Src::kPhilox) {
-        sim_pair_f64_fast<SCHEME, JUMPS, PAYOFF>(d, src, anti, ea, eb);
+        sim_pair_f64_fast<SCHEME, JUMPS, PAYOFF, GROUP>(d, src, anti, ea, eb, lds);
         return;
     }
     double la = d.ln_spot, lb = d.ln_spot, va = d.v0, vb = d.v0;
@@ -760,10 +881,12 @@ __device__ __forceinline__ void sim_pair_f64(const McDesc& d, const Src& src, bo
 // --------------------------------------------- pair simulation, FP32 path
 // ANTI: the pair has a sibling (QE's loop is compiled once per case; the
 // Euler loop always computes the sibling, and `anti` decides whether its
-// result is kept).
-template <int SCHEME, bool JUMPS, int PAYOFF, bool ANTI, class Src>
+// result is kept).  GROUP: the block regroups its pairs by variance every 8
+// steps through lds (QE, Philox mode, no deferred jumps; every thread of
+// the block steps).
+template <int SCHEME, bool JUMPS, int PAYOFF, bool ANTI, bool GROUP, class Src>
 __device__ __forceinline__ void sim_pair_f32_loop(const McDesc& d, const Src& src, bool anti, PathState& ea,
-                                                  PathState& eb) {
+                                                  PathState& eb, void* lds) {
     double ya = 0.0, yb = 0.0;
     // QE only ever reads v+ = max(v, 0) and never produces a negative v; the
     // Euler step's handling of a negative v0 is below
@@ -844,7 +967,60 @@ __device__ __forceinline__ void sim_pair_f32_loop(const McDesc& d, const Src& sr
 
     if constexpr (Src::kPhilox) {
         constexpr bool kDefer = JUMPS && PAYOFF == kPayEuropean;
-        philox_steps<true, JUMPS, kDefer, kMixed>(d, src.st, step, &jsum, fold);
+        static_assert(!GROUP || (SCHEME == kSchemeQE && !kDefer), "regrouping: QE without deferred jumps");
+        GroupLds* g = static_cast<GroupLds*>(lds);
+        uint32_t home = threadIdx.x;
+        // the words of a pair's state at a block end (pa = pb = 0, ma = mb =
+        // -inf there; fa, fb follow from ya, yb)
+        constexpr int W = 8 + (PAYOFF == kPayAsian || PAYOFF == kPayUpAndOut ? 4 : 0);
+        auto move = [&](uint32_t dst, PairStream& st) {
+            uint32_t r[W] = {};
+            r[0] = __float_as_uint(va);
+            r[1] = __float_as_uint(vb);
+            put_f64(r + 2, ya);
+            put_f64(r + 4, yb);
+            r[6] = st.plo;
+            r[7] = home;
+            if constexpr (PAYOFF == kPayAsian) {
+                r[8] = __float_as_uint(suma);
+                r[9] = __float_as_uint(sumb);
+            }
+            if constexpr (PAYOFF == kPayUpAndOut) {
+                put_f64(r + 8, mxa);
+                put_f64(r + 10, mxb);
+            }
+            group_move(g, dst, r);
+            va = __uint_as_float(r[0]);
+            vb = __uint_as_float(r[1]);
+            ya = get_f64(r + 2);
+            yb = get_f64(r + 4);
+            st.plo = r[6];
+            home = r[7];
+            if constexpr (PAYOFF == kPayAsian) {
+                suma = __uint_as_float(r[8]);
+                sumb = __uint_as_float(r[9]);
+                fa = (float)ya;
+                fb = (float)yb;
+            }
+            if constexpr (PAYOFF == kPayUpAndOut) {
+                mxa = get_f64(r + 8);
+                mxb = get_f64(r + 10);
+            }
+        };
+        auto block_end = [&](uint32_t s, PairStream& st) {
+            fold(s);
+            if constexpr (GROUP) {
+                const float vk = ANTI ? fminf(va, vb) : va;
+                move(group_dest(g, vk < d.qe_vlow, (blockIdx.x + (s >> 3)) & 3u), st);
+            }
+        };
+        philox_steps<true, JUMPS, kDefer, kMixed>(d, src.st, step, &jsum, block_end);
+        if constexpr (GROUP) {  // every pair back to its home lane, its last partial block folded
+            fold(N);
+            PairStream st = src.st;
+            __syncthreads();
+            move(home, st);
+        }
     } else {
         for (uint32_t i = 0; i < N; ++i) {
             double za, zb, j;
@@ -887,13 +1063,14 @@ __device__ __forceinline__ void sim_pair_f32_loop(const McDesc& d, const Src& sr
     if (anti) finish(eb, yb, vb, sumb, mxb);
 }
 
-template <int SCHEME, bool JUMPS, int PAYOFF, class Src>
+// GROUP: as sim_pair_f32_loop (lds: the block's LDS).
+template <int SCHEME, bool JUMPS, int PAYOFF, bool GROUP = false, class Src>
 __device__ __forceinline__ void sim_pair_f32(const McDesc& d, const Src& src, bool anti, PathState& ea,
-                                             PathState& eb) {
+                                             PathState& eb, void* lds = nullptr) {
     if (SCHEME == kSchemeQE && !anti)
-        sim_pair_f32_loop<SCHEME, JUMPS, PAYOFF, false>(d, src, false, ea, eb);
+        sim_pair_f32_loop<SCHEME, JUMPS, PAYOFF, false, GROUP>(d, src, false, ea, eb, lds);
     else
-        sim_pair_f32_loop<SCHEME, JUMPS, PAYOFF, true>(d, src, anti, ea, eb);
+        sim_pair_f32_loop<SCHEME, JUMPS, PAYOFF, true, GROUP>(d, src, anti, ea, eb, lds);
     if constexpr (PAYOFF == kPayUpAndOut) {
         // With the call's FP64 barrier flag, the barrier decision near B is
         // FP64's: the pair is simulated again on the same draws by the FP64
@@ -959,6 +1136,7 @@ constexpr int kEpiWsumOff = (kEpiStrikes * kEpiRow * 8 + 15) / 16;
 constexpr int kTileLds = kMaxStrikes * 4 > kEpiWsumOff + kEpiStrikes * 4 ? kMaxStrikes * 4
                                                                           : kEpiWsumOff + kEpiStrikes * 4;
 static_assert(kBlock == 256 && kEpiStrikes == 8, "thread t sums threads 8 (t >> 3) .. +7 of row t & 7");
+static_assert(sizeof(GroupLds) <= sizeof(double2) * kTileLds, "the regrouping's rows fit the tile LDS");
 
 // One tile: kBlock * PPT pairs of one request.  Thread t simulates pairs
 // tile_base + j*kBlock + t (j < PPT), then for every strike the block sums
@@ -982,17 +1160,25 @@ __device__ __forceinline__ void price_tile(const McDesc& d, uint32_t tile, const
     // of the step code) and hands its result over through selects.
     PathState ea[PPT], eb[PPT];
     bool valid[PPT];
+    // QE in Philox mode regroups the block's pairs by variance (see
+    // group_dest), so every thread steps, also those past the last pair of a
+    // ragged tile: their draws are Philox words of pair indices no request
+    // owns, and their results are dropped.
+    using Src = decltype(make_src(d, uint64_t(0), uint32_t(0)));
+    constexpr bool kGroup = SCHEME == kSchemeQE && Src::kPhilox && !(JUMPS && PAYOFF == kPayEuropean);
 #pragma nounroll
     for (int j = 0; j < PPT; ++j) {
         const uint64_t pair = base + (uint64_t)j * kBlock + (uint64_t)tid;
         const bool ok = pair < d.n_pairs;
         PathState a, b;
-        if (ok) {
+        if (ok || kGroup) {
             const auto src = make_src(d, pair, pair_hi);
             if constexpr (sizeof(Real) == 8)
-                sim_pair_f64<SCHEME, JUMPS, PAYOFF>(d, src, anti, a, b);
+                sim_pair_f64<SCHEME, JUMPS, PAYOFF, kGroup>(d, src, anti, a, b, lds);
             else
-                sim_pair_f32<SCHEME, JUMPS, PAYOFF>(d, src, anti, a, b);
+                sim_pair_f32<SCHEME, JUMPS, PAYOFF, kGroup>(d, src, anti, a, b, lds);
+        }
+        if (ok) {
             if (sink.payoff) {
                 const double k0 = strikes[d.strike_off];
                 sink.payoff[pair * 2] = payoff_value(d, a, k0);
@@ -1013,6 +1199,9 @@ __device__ __forceinline__ void price_tile(const McDesc& d, uint32_t tile, const
             }
         }
     }
+    // the strike loop reuses the LDS the regrouping used: every thread has
+    // taken its pair back first
+    if constexpr (kGroup) __syncthreads();
 
     const int lane = tid & 63, wave = tid >> 6;
     const uint32_t ns = d.n_strikes;
