@@ -177,8 +177,11 @@ int make_desc(const Request& r, McDesc& d) {
                     r.cfg.antithetic ? " pair (antithetic needs n_paths >= 2)" : "");
     if (n_pairs > kMaxPairs) return fail(OPE_ERR_INVALID_ARGUMENT, "n_paths too large");
     const bool qe = r.cfg.scheme == OPE_SCHEME_QUADRATIC_EXPONENTIAL;
-    if (qe && !(hp.sigma > 0.0 && hp.kappa > 0.0))
-        return fail(OPE_ERR_INVALID_ARGUMENT, "QE scheme needs sigma > 0 and kappa > 0");
+    // theta < 0 would make the conditional mean m negative, which the QE
+    // steps of the Philox kernels (v >= 0 throughout, m w = sqrt(m^2 - s2/2))
+    // do not represent
+    if (qe && !(hp.sigma > 0.0 && hp.kappa > 0.0 && hp.theta >= 0.0))
+        return fail(OPE_ERR_INVALID_ARGUMENT, "QE scheme needs sigma > 0, kappa > 0 and theta >= 0");
     if (r.has_jump_params) {
         if (!finite_all({r.lambda, r.mu_j, r.sigma_j}))
             return fail(OPE_ERR_INVALID_ARGUMENT, "non-finite jump parameters");

@@ -479,45 +479,52 @@ __device__ __forceinline__ float ncdf_neg_abs_f32(float z) {
 }
 
 // QE variance step in FP32 (Andersen 2008, psi_c = 1.5) for one sibling, in
-// terms of the conditional mean m and u = psi/2 = s2h/m^2, s2h = s2/2 (the
-// host halves its coefficients); psi > 1.5 <=> u > 0.75.  u is floored at
-// the reference's psi floor (psi >= 1e-12) and capped at 1e30, so that a
-// vanishing m^2 leaves every product finite; with m = 0 (theta = 0 and v = 0)
-// u = 0 * inf is NaN, which the floor turns into 5e-13 (fmaxf returns its
-// other operand), and v' = 0 as in the reference.
+// terms of the conditional mean m, s2h = s2/2 (the host halves its
+// coefficients) and u = psi/2 = s2h/m^2; psi > 1.5 <=> u > 0.75 <=> s2h >
+// 0.75 m^2.
 // * quadratic: with w = sqrt(1 - u), 2/psi = 1/u and
 //   b^2 = 2/psi - 1 + sqrt(2/psi (2/psi - 1)) = w (1 + w)/u,
 //   a = m/(1 + b^2) = m u/(1 + w), so a b^2 = m w and
-//   v' = a (b + z)^2 = m w (1 + z/b)^2 with 1/b = sqrt(u/(w (1 + w)))
-//   = u rsqrt(u w (1 + w)): three transcendentals, no division, and no
-//   cancellation for u in [0, 0.75];
+//   v' = a (b + z)^2 = m w (1 + z/b)^2.  With D = m^2 - s2h = (m w)^2
+//   (m >= 0), m w = sqrt(D) and w (1 + w)/u = m w (m + m w)/s2h, so
+//   1/b = sqrt(s2h/(m w (m + m w))) = s2h rsqrt(s2h (m w m + D)):
+//   two transcendentals, no division and no u, and no cancellation for u in
+//   [0, 0.75] (D >= m^2/4).  FLT_MIN added to the rsqrt's argument keeps
+//   s2h = 0 (sigma = 0, or theta = v = 0) at 1/b = 0, v' = m, instead of
+//   0 * inf; the reference's psi floor (psi >= 1e-12) moves v' by at most
+//   ~1e-6 relative there.  Below m ~ 1e-7 the argument underflows and 1/b
+//   loses precision, where v' itself is ~1e-7 and the price does not see it;
 // * exponential: 1 - p = 2/(psi + 1) = 1/(u + 1/2) and 1/beta = m (u + 1/2),
 //   and with phic = 1 - ncdf(z): v' = 0 when u_z <= p (phic (u + 1/2) >= 1),
 //   else ln((1 - p)/phic)/beta = m (u + 1/2) max(-ln((u + 1/2) phic), 0),
-//   which avoids the cancellation of 1 - ncdf(z).
-// qe_sib_f32 returns m, u and the quadratic branch's v'; on exponential
-// lanes that v' is meaningless (sqrt of 1 - u < 0) and the caller replaces it.
+//   which avoids the cancellation of 1 - ncdf(z).  Only this branch needs
+//   u, floored at the reference's psi floor and capped at 1e30 so that a
+//   vanishing m^2 leaves every product finite.
+// qe_sib_f32 returns m, s2h and the quadratic branch's v'; on exponential
+// lanes that v' is meaningless (sqrt of D < m^2/4) and the caller replaces it.
 struct QeSib {
-    float m, u, vq;
+    float m, s2h, vq;
     bool ex;
 };
 
 __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z, float m0, float s0) {
     QeSib o;
     o.m = fmaf(d.qe_m1, v, m0);
-    const float s2h = fmaf(d.qe_s1, v, s0);
-    o.u = fminf(fmaxf(s2h * hw_rcp(o.m * o.m), 5e-13f), 1e30f);
-    o.ex = o.u > 0.75f;
-    const float w = hw_sqrt(1.0f - o.u);
-    const float ib = o.u * hw_rsq(o.u * fmaf(w, w, w));  // 1/b
+    o.s2h = fmaf(d.qe_s1, v, s0);
+    const float mm = o.m * o.m;
+    o.ex = o.s2h > 0.75f * mm;
+    const float D = mm - o.s2h;
+    const float mw = hw_sqrt(D);  // m w
+    const float ib = o.s2h * hw_rsq(fmaf(o.s2h, fmaf(mw, o.m, D), 0x1p-126f));  // 1/b
     const float f = fmaf(z, ib, 1.0f);
-    o.vq = (o.m * w) * (f * f);
+    o.vq = mw * (f * f);
     return o;
 }
 
 // The exponential branch's v' for tail probability phic = 1 - ncdf(z).
 __device__ __forceinline__ float qe_exp_f32(const QeSib& s, float phic) {
-    const float uh = s.u + 0.5f;
+    const float u = fminf(fmaxf(s.s2h * hw_rcp(s.m * s.m), 5e-13f), 1e30f);
+    const float uh = u + 0.5f;
     return (s.m * uh) * fmaxf(hw_log2(uh * phic) * -0.69314718055994531f, 0.0f);
 }
 
@@ -637,30 +644,34 @@ __device__ __forceinline__ double euler_f64_fast(const McDesc& d, double& v, dou
     return inc;
 }
 
-// The QE variance step of one sibling in FP64, as qe_sib_f32: m, u = psi/2
-// (floored at the psi floor 1e-12, capped at 1e30) and the quadratic
-// branch's v' = m w (1 + z/b)^2, w = sqrt(1 - u), 1/b = u rsqrt(u w (1 + w)).
+// The QE variance step of one sibling in FP64, as qe_sib_f32: m, s2h and the
+// quadratic branch's v' = m w (1 + z/b)^2 with m w = sqrt(D), D = m^2 - s2h,
+// and 1/b = s2h rsqrt(s2h (m w m + D)) (DBL_MIN added against 0 * inf at
+// s2h = 0); the exponential branch forms u = psi/2 (floored at the psi floor
+// 1e-12, capped at 1e30).
 struct QeSib64 {
-    double m, u, vq;
+    double m, s2h, vq;
     bool ex;
 };
 
 __device__ __forceinline__ QeSib64 qe_sib_f64(const McDesc& d, double v, double z) {
     QeSib64 o;
     o.m = fma(d.ekt, v, d.qd_m0);
-    const double s2h = fma(d.qd_s1, v, d.qd_s0);
-    o.u = fmin(fmax(s2h * rcp_f64(o.m * o.m), 5e-13), 1e30);
-    o.ex = o.u > 0.75;
-    const double w = sqrt(1.0 - o.u);
-    const double ib = o.u * rsqrt_f64(o.u * fma(w, w, w));
+    o.s2h = fma(d.qd_s1, v, d.qd_s0);
+    const double mm = o.m * o.m;
+    o.ex = o.s2h > 0.75 * mm;
+    const double D = mm - o.s2h;
+    const double mw = sqrt(D);
+    const double ib = o.s2h * rsqrt_f64(fma(o.s2h, fma(mw, o.m, D), 0x1p-1022));
     const double f = fma(z, ib, 1.0);
-    o.vq = (o.m * w) * (f * f);
+    o.vq = mw * (f * f);
     return o;
 }
 
 // The exponential branch's v' for tail probability phic = 1 - ncdf(z).
 __device__ __forceinline__ double qe_exp_f64(const QeSib64& s, double phic) {
-    const double uh = s.u + 0.5;
+    const double u = fmin(fmax(s.s2h * rcp_f64(s.m * s.m), 5e-13), 1e30);
+    const double uh = u + 0.5;
     return (s.m * uh) * fmax(-log_f64(uh * phic), 0.0);
 }
 

@@ -92,6 +92,7 @@ def _hp(ope, **kw):
     dict(expiry=-0.5),
     dict(hp=dict(sigma=0.0), cfg=dict(scheme=1)),      # QE divides by sigma (mc.rs:233-235)
     dict(hp=dict(kappa=0.0), cfg=dict(scheme=1)),      # and by kappa (mc.rs:293-294)
+    dict(hp=dict(theta=-0.01), cfg=dict(scheme=1)),    # negative conditional mean (mc.rs:291)
     dict(payoff=(5, 0)),                               # unknown payoff kind
     dict(payoff=(0, 3)),                               # unknown option type
     dict(jumps=(-0.1, 0.0, 0.1)),
