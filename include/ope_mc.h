@@ -331,8 +331,9 @@ int ope_debug_philox(uint64_t seed, const uint32_t *ctr, uint32_t n, uint32_t *o
 /* The FP64 Philox-mode path's elementary functions on the device, for
  * accuracy checks (no reference counterpart): fn 0 natural log, 1 exp,
  * 2 / 3 sine / cosine of 2 pi x 2^-23 for integer x in [0, 2^23) (the
- * Box-Muller angle), 4 reciprocal, 5 reciprocal square root; out[i] =
- * f(x[i]). */
+ * Box-Muller angle), 4 reciprocal, 5 reciprocal square root, 6 square
+ * root (x = 0 or a positive normal), 7 normal tail Phi(-|x|) for
+ * |x| <= 6.8; out[i] = f(x[i]). */
 int ope_debug_fp64_math(int32_t fn, const double *x, uint32_t n, double *out);
 
 /* The Philox-mode draws of pairs [pair0, pair0+n_pairs) over n_steps steps as

@@ -816,7 +816,15 @@ int ope_debug_philox(uint64_t seed, const uint32_t* ctr, uint32_t n, uint32_t* o
 
 int ope_debug_fp64_math(int32_t fn, const double* x, uint32_t n, double* out) {
     if ((!x || !out) && n) return fail(OPE_ERR_INVALID_ARGUMENT, "null argument");
-    if (fn < 0 || fn > 5) return fail(OPE_ERR_INVALID_ARGUMENT, "unknown function %d", fn);
+    if (fn < 0 || fn > 7) return fail(OPE_ERR_INVALID_ARGUMENT, "unknown function %d", fn);
+    if (fn == 6)
+        for (uint32_t i = 0; i < n; ++i)
+            if (!(x[i] >= 0.0 && x[i] <= 1.79769313486231570e308))
+                return fail(OPE_ERR_INVALID_ARGUMENT, "square root argument %u is negative or not finite", i);
+    if (fn == 7)
+        for (uint32_t i = 0; i < n; ++i)
+            if (!(std::fabs(x[i]) <= 6.8))
+                return fail(OPE_ERR_INVALID_ARGUMENT, "normal tail argument %u is outside [-6.8, 6.8]", i);
     if (fn == 2 || fn == 3)
         for (uint32_t i = 0; i < n; ++i)
             if (!(x[i] >= 0.0 && x[i] < 8388608.0 && x[i] == std::floor(x[i])))

@@ -103,7 +103,7 @@ constexpr double kLn2 = 0.6931471805599453;
 // normals up to rounding.
 __device__ __forceinline__ void box_muller_f64(uint32_t w0, uint32_t w1, double& za, double& zb) {
     const double u1 = ((double)w0 + 0.5) * 0x1p-32;
-    const double r = sqrt(-2.0 * log_f64(u1));
+    const double r = sqrt_f64(-2.0 * log_f64(u1));
     double s, c;
     sincos_rev23(w1 >> kGateBits, s, c);
     za = r * c;
@@ -182,7 +182,7 @@ __device__ __forceinline__ auto jump_of_step(const McDesc& d, const PairStream& 
         } else {
             double z, unused;
             box_muller_f64(w.x, w.y, z, unused);
-            j = d.mu_j * (double)n + d.sigma_j * sqrt((double)n) * z;
+            j = d.mu_j * (double)n + d.sigma_j * sqrt_f64((double)n) * z;
         }
     }
     return j;
@@ -638,7 +638,7 @@ struct ReplaySource {
 // enters as 0 with b0 + v0 for its first step, as in the FP32 path).  sa =
 // sqrt(dt) z1 and sb = sigma sqrt(dt) z2 are per-pair (negated for sibling b).
 __device__ __forceinline__ double euler_f64_fast(const McDesc& d, double& v, double sa, double sb, double b0) {
-    const double sv = sqrt(v);
+    const double sv = sqrt_f64(v);
     const double inc = fma(sv, sa, fma(d.ed_a1, v, d.ed_a0));
     v = fmax(fma(sv, sb, fma(d.ed_c1, v, b0)), 0.0);
     return inc;
@@ -661,7 +661,7 @@ __device__ __forceinline__ QeSib64 qe_sib_f64(const McDesc& d, double v, double 
     const double mm = o.m * o.m;
     o.ex = o.s2h > 0.75 * mm;
     const double D = mm - o.s2h;
-    const double mw = sqrt(D);
+    const double mw = sqrt_f64(D);
     const double ib = o.s2h * rsqrt_f64(fma(o.s2h, fma(mw, o.m, D), 0x1p-1022));
     const double f = fma(z, ib, 1.0);
     o.vq = mw * (f * f);
@@ -689,16 +689,16 @@ __device__ __forceinline__ void qe_pair_f64_fast(const McDesc& d, bool anti, dou
     if (a.ex || (anti && b.ex)) {
         // e = ncdf(-|za|) (math.rs:16-19); 1 - ncdf(z) is e for the sibling
         // whose z >= 0 and 1 - e for the other
-        const double e = 0.5 * erfc(fabs(za) * 0.7071067811865476), ome = 1.0 - e;
+        const double e = ncdf_neg_abs_f64(za), ome = 1.0 - e;
         const bool pos = za >= 0.0;
         if (a.ex) na = qe_exp_f64(a, pos ? e : ome);
         if (anti && b.ex) nb = qe_exp_f64(b, pos ? ome : e);
     }
     const double zs = d.qd_sq3 * zb;
-    dya = fma(sqrt(va + na), zs, fma(d.k2, na, fma(d.k1, va, d.qe_a0)));
+    dya = fma(sqrt_f64(va + na), zs, fma(d.k2, na, fma(d.k1, va, d.qe_a0)));
     va = na;
     if (anti) {
-        dyb = fma(sqrt(vb + nb), -zs, fma(d.k2, nb, fma(d.k1, vb, d.qe_a0)));
+        dyb = fma(sqrt_f64(vb + nb), -zs, fma(d.k2, nb, fma(d.k1, vb, d.qe_a0)));
         vb = nb;
     } else {
         dyb = 0.0;

@@ -84,7 +84,8 @@ __global__ void k_debug_draws(const McDesc* __restrict__ desc, uint64_t pair0, u
 
 // The FP64 Philox path's elementary functions (fp64_math.h) on given inputs:
 // fn 0 log_f64, 1 exp_f64, 2 / 3 sine / cosine of sincos_rev23 (x holds the
-// 23-bit integer angle), 4 rcp_f64.
+// 23-bit integer angle), 4 rcp_f64, 5 rsqrt_f64, 6 sqrt_f64, 7
+// ncdf_neg_abs_f64.
 __global__ void k_debug_fp64_math(int fn, const double* __restrict__ x, uint32_t n, double* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -100,8 +101,12 @@ __global__ void k_debug_fp64_math(int fn, const double* __restrict__ x, uint32_t
         r = fn == 2 ? sn : cs;
     } else if (fn == 4) {
         r = rcp_f64(v);
-    } else {
+    } else if (fn == 5) {
         r = rsqrt_f64(v);
+    } else if (fn == 6) {
+        r = sqrt_f64(v);
+    } else {
+        r = ncdf_neg_abs_f64(v);
     }
     out[i] = r;
 }

@@ -435,7 +435,7 @@ def debug_philox(seed: int, ctr: np.ndarray) -> np.ndarray:
 def debug_fp64_math(fn: int, x) -> np.ndarray:
     """The FP64 path's elementary functions on the GPU: fn 0 log, 1 exp,
     2 / 3 sin / cos of 2 pi x 2^-23 (integer x), 4 reciprocal, 5 reciprocal
-    square root."""
+    square root, 6 square root, 7 normal tail Phi(-|x|) for |x| <= 6.8."""
     x = np.ascontiguousarray(x, dtype=np.float64).ravel()
     out = np.zeros_like(x)
     P = C.POINTER(C.c_double)

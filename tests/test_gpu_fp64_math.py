@@ -1,6 +1,6 @@
 """The FP64 Philox-mode path's elementary functions (fp64_math.h) against
 exact values (mpmath at 60 digits, rounded once): within 2 ulp over the
-ranges the path uses and beyond them.  The replay kernels do not use these
+ranges the path uses and beyond them (the normal tail within 8 ulp).  The replay kernels do not use these
 (they call the device library in the reference's order)."""
 import math
 
@@ -91,6 +91,39 @@ def test_reciprocal_square_root(ope):
     err = ulp_err(got, [1 / mpmath.sqrt(mpmath.mpf(v)) for v in x])
     print(f"rsqrt: max {err.max():.2f} ulp", flush=True)
     assert err.max() <= 1.0
+
+
+def test_square_root(ope):
+    rng = np.random.default_rng(6)
+    # QE's m w = sqrt(m^2 - s^2/2) and variance terms, Euler's sqrt(v),
+    # the Box-Muller radius, the jump count, and the full normal range
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 3000), rng.uniform(0.0, 1e-3, 2000),
+                        rng.uniform(0.0, 46.0, 2000), np.arange(1.0, 65.0),
+                        np.array([0.0, 1.0, 4.0, 0.25, 2.0, 2.2250738585072014e-308])])
+    got = ope.debug_fp64_math(6, x)
+    err = ulp_err(got[x > 0], [mpmath.sqrt(mpmath.mpf(v)) for v in x[x > 0]])
+    print(f"sqrt: max {err.max():.2f} ulp, {np.mean(err == 0):.4f} correctly rounded", flush=True)
+    assert np.all(got[x == 0.0] == 0.0)
+    sq = np.array([1.0, 4.0, 0.25, 4096.0])
+    assert np.all(ope.debug_fp64_math(6, sq) == np.sqrt(sq))
+    assert err.max() <= 1.0
+
+
+def test_normal_tail(ope):
+    rng = np.random.default_rng(7)
+    # every Box-Muller normal lies in [-6.77, 6.77]: the QE exponential branch's
+    # ncdf(-|z|), the small tail probability that must keep its relative accuracy
+    z = np.concatenate([rng.uniform(-6.8, 6.8, 4000), np.linspace(0.0, 6.8, 681), np.array([0.0, -6.8, 6.8])])
+    got = ope.debug_fp64_math(7, z)
+    exact = [mpmath.erfc(abs(mpmath.mpf(v)) / mpmath.sqrt(2)) / 2 for v in z]
+    err = ulp_err(got, exact)
+    # about 3e-16 absolute at the worst points (values just below 1/2, where
+    # the roundings of t and of e^{-z^2/2} in [1/2, 1) count double)
+    print(f"ncdf(-|z|): max {err.max():.2f} ulp", flush=True)
+    assert err.max() <= 8.0
+    assert np.array_equal(got, ope.debug_fp64_math(7, -z))  # a function of |z|
+    with pytest.raises(ope.OpeError):
+        ope.debug_fp64_math(7, np.array([6.9]))
 
 
 def test_angle_must_be_an_integer(ope):
