@@ -33,6 +33,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
 
 UNITS = [
     ("mc_kernels.hip", ["-ffp-contract=on"]),
+    ("mc_direct.hip", ["-ffp-contract=on"]),
     ("mc_server.hip", ["-ffp-contract=on"]),
     ("mc_noalloc.hip", ["-ffp-contract=on"]),
     ("mc_replay.hip", ["-ffp-contract=off"]),

@@ -5,7 +5,9 @@
 //   make_desc     the loop invariants of mc.rs:153-157 and run_path's QE
 //                 coefficients (mc.rs:230-238), computed once in FP64
 //   plan          pairs -> tiles of kBlock*ppt pairs -> FP64 partial slots
-//   run_plan      one pricing launch + one slot reduction launch
+//   run_plan      direct launches (one request per launch group, descriptor
+//                 and strikes in the kernel arguments, the reduction inside
+//                 the launch), or staged launches + one slot reduction launch
 //   finish        mean and std_error exactly as mc.rs:198-204
 #include <hip/hip_runtime.h>
 
@@ -112,6 +114,9 @@ struct Context {
     // no device-to-host copy per call
     HostBuf h_out{nullptr, 0, hipHostMallocCoherent};
     DevBuf rz_a, rz_b, rz_j, path_pay, path_state;
+    // completion counters of the direct launches, one per stream: zero
+    // between calls (each launch's last block clears its own)
+    DevBuf done;
     DevBuf dbg_in, dbg_out, dbg_a, dbg_b, dbg_j, dbg_n;
 };
 
@@ -138,6 +143,12 @@ int ensure_init(Context& c) {
             c.init_status = fail(OPE_ERR_NO_DEVICE, "HIP initialisation failed: %s", hipGetErrorString(e));
             return c.init_status;
         }
+    }
+    if ((e = c.done.ensure((1 + kAuxStreams) * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMemsetAsync(c.done.p, 0, (1 + kAuxStreams) * sizeof(uint32_t), c.stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c.stream)) != hipSuccess) {
+        c.init_status = fail(OPE_ERR_NO_DEVICE, "HIP initialisation failed: %s", hipGetErrorString(e));
+        return c.init_status;
     }
     c.init_status = OPE_OK;
     return OPE_OK;
@@ -320,9 +331,10 @@ struct Plan {
     uint64_t n_slots = 0;
     uint32_t n_out = 0;
     // Launch groups over `tiles`: each group is one kernel launch; code ==
-    // kNumCodes marks the mixed kernel.
+    // kNumCodes marks the mixed kernel; req is the group's request when it
+    // has only one (else ~0u).
     struct Group {
-        uint32_t code, first, count;
+        uint32_t code, first, count, req;
     };
     std::vector<Group> groups;
 };
@@ -405,7 +417,9 @@ int plan(std::vector<Job>& in, Plan& p) {
     for (uint32_t i : order) {
         const uint32_t code = grouped ? config_code(p.descs[i]) : kNumCodes;
         if (p.groups.empty() || p.groups.back().code != code)
-            p.groups.push_back(Plan::Group{code, (uint32_t)p.tiles.size(), 0});
+            p.groups.push_back(Plan::Group{code, (uint32_t)p.tiles.size(), 0, i});
+        else
+            p.groups.back().req = ~0u;
         for (uint32_t t = 0; t < p.descs[i].n_tiles; ++t) p.tiles.push_back(make_uint2(i, t));
         p.groups.back().count += p.descs[i].n_tiles;
     }
@@ -426,7 +440,59 @@ struct RawOut {
     double kernel_ms = 0.0;  // pricing kernel alone
 };
 
+// A plan runs as direct launches when each of its groups is one request of
+// one configuration with at most kDirectStrikes strikes, one group per
+// stream: its descriptor and strikes go in the kernel arguments and the
+// launch reduces its own slots, so the call makes no upload and no
+// reduction launch.  The arithmetic and the reduction order are those of
+// the staged launches below.
+bool is_direct(const Plan& p) {
+    if (p.groups.empty() || p.groups.size() > (size_t)(1 + kAuxStreams)) return false;
+    for (const Plan::Group& g : p.groups)
+        if (g.code >= kNumCodes || g.req == ~0u || p.descs[g.req].n_strikes > kDirectStrikes) return false;
+    return true;
+}
+
+int run_direct(Context& c, const Plan& p, int fp32, RawOut& r) {
+    hipStream_t s = c.stream;
+    const size_t b_out = p.n_out * sizeof(double2);
+    HIP_TRY(c.h_out.ensure(b_out));
+    HIP_TRY(c.slots.ensure(p.n_slots * sizeof(double2)));
+    HIP_TRY(hipEventRecord(c.ev0, s));
+    const int n = (int)p.groups.size();
+    auto lane_stream = [&](int lane) { return lane == 0 ? s : c.aux[lane - 1]; };
+    for (int gi = 0; gi < n; ++gi) {
+        const McDesc& d = p.descs[p.groups[gi].req];
+        DirectLaunch L;
+        L.args.d = d;
+        L.args.d.strike_off = 0;
+        for (uint32_t k = 0; k < kDirectStrikes; ++k)
+            L.args.strikes[k] = k < d.n_strikes ? p.strikes[d.strike_off + k] : 0.0;
+        L.slots = c.slots.as<double2>();
+        L.done = c.done.as<uint32_t>() + gi;
+        L.out = static_cast<double2*>(c.h_out.p);
+        L.fp32 = fp32;
+        if (gi > 0) HIP_TRY(hipStreamWaitEvent(c.aux[gi - 1], c.ev0, 0));
+        HIP_TRY(launch_direct(L, lane_stream(gi)));
+    }
+    hipStream_t fin = lane_stream(n - 1);
+    for (int i = 0; i < n - 1; ++i) {
+        HIP_TRY(hipEventRecord(c.join[i], lane_stream(i)));
+        HIP_TRY(hipStreamWaitEvent(fin, c.join[i], 0));
+    }
+    HIP_TRY(hipEventRecord(c.ev1, fin));
+    HIP_TRY(hipStreamSynchronize(fin));
+    r.sums.resize(p.n_out);
+    std::memcpy(r.sums.data(), c.h_out.p, b_out);
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    r.device_ms = ms;
+    r.kernel_ms = ms;  // the path kernels, each with its last block's reduction
+    return OPE_OK;
+}
+
 int run_plan(Context& c, const Plan& p, int fp32, RawOut& r) {
+    if (is_direct(p)) return run_direct(c, p, fp32, r);
     hipStream_t s = c.stream;
     // descriptors, tiles, strikes and jobs: one pinned staging area, one copy
     auto at = [](size_t x) { return (x + 255) & ~size_t(255); };

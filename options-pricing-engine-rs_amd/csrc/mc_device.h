@@ -498,8 +498,9 @@ __device__ __forceinline__ float ncdf_neg_abs_f32(float z) {
 //   and with phic = 1 - ncdf(z): v' = 0 when u_z <= p (phic (u + 1/2) >= 1),
 //   else ln((1 - p)/phic)/beta = m (u + 1/2) max(-ln((u + 1/2) phic), 0),
 //   which avoids the cancellation of 1 - ncdf(z).  Only this branch needs
-//   u, floored at the reference's psi floor and capped at 1e30 so that a
-//   vanishing m^2 leaves every product finite.
+//   u; its lanes have s2h > 0.75 m^2 >= 0, so u > 0.75 (the psi floor
+//   cannot act) or u = s2h / 0 = inf, capped at 1e30 so that a vanishing
+//   m^2 leaves every product finite.
 // qe_sib_f32 returns m, s2h and the quadratic branch's v'; on exponential
 // lanes that v' is meaningless (sqrt of D < m^2/4) and the caller replaces it.
 struct QeSib {
@@ -523,7 +524,7 @@ __device__ __forceinline__ QeSib qe_sib_f32(const McDesc& d, float v, float z, f
 
 // The exponential branch's v' for tail probability phic = 1 - ncdf(z).
 __device__ __forceinline__ float qe_exp_f32(const QeSib& s, float phic) {
-    const float u = fminf(fmaxf(s.s2h * hw_rcp(s.m * s.m), 5e-13f), 1e30f);
+    const float u = fminf(s.s2h * hw_rcp(s.m * s.m), 1e30f);
     const float uh = u + 0.5f;
     return (s.m * uh) * fmaxf(hw_log2(uh * phic) * -0.69314718055994531f, 0.0f);
 }
@@ -647,8 +648,8 @@ __device__ __forceinline__ double euler_f64_fast(const McDesc& d, double& v, dou
 // The QE variance step of one sibling in FP64, as qe_sib_f32: m, s2h and the
 // quadratic branch's v' = m w (1 + z/b)^2 with m w = sqrt(D), D = m^2 - s2h,
 // and 1/b = s2h rsqrt(s2h (m w m + D)) (DBL_MIN added against 0 * inf at
-// s2h = 0); the exponential branch forms u = psi/2 (floored at the psi floor
-// 1e-12, capped at 1e30).
+// s2h = 0); the exponential branch forms u = psi/2 (> 0.75 there, capped at
+// 1e30).
 struct QeSib64 {
     double m, s2h, vq;
     bool ex;
@@ -670,7 +671,7 @@ __device__ __forceinline__ QeSib64 qe_sib_f64(const McDesc& d, double v, double 
 
 // The exponential branch's v' for tail probability phic = 1 - ncdf(z).
 __device__ __forceinline__ double qe_exp_f64(const QeSib64& s, double phic) {
-    const double u = fmin(fmax(s.s2h * rcp_f64(s.m * s.m), 5e-13), 1e30);
+    const double u = fmin(s.s2h * rcp_f64(s.m * s.m), 1e30);
     const double uh = u + 0.5;
     return (s.m * uh) * fmax(-log_f64(uh * phic), 0.0);
 }

@@ -27,6 +27,23 @@ struct TileLaunch {
     uint32_t code;
 };
 
+// A direct launch: one request whose descriptor and strikes travel in the
+// kernel arguments (no upload), tile = block index, and the block that
+// finishes the last tile sums every strike's slots in tile order into `out`
+// (block_sum_slots, as k_reduce: the same bits), then clears `done`.
+constexpr uint32_t kDirectStrikes = 16;
+struct DirectArgs {
+    McDesc d;                        // strike_off = 0: the strikes below
+    double strikes[kDirectStrikes];
+};
+struct DirectLaunch {
+    DirectArgs args;
+    double2* slots;        // device
+    uint32_t* done;        // device, 0 before the launch; the last block resets it
+    double2* out;          // (sum, sum_sq) per strike at args.d.out_off; host-coherent memory
+    int fp32;
+};
+
 struct ReplayLaunch {
     const McDesc* desc;    // device, one request, ppt == 1
     uint32_t n_tiles;
@@ -45,6 +62,7 @@ hipError_t launch_tiles(const TileLaunch& L, hipStream_t s);
 hipError_t launch_reduce(const McDesc* descs, const uint2* jobs, uint32_t n_jobs, const double2* slots,
                          double2* out, hipStream_t s);
 hipError_t launch_replay(const ReplayLaunch& L, hipStream_t s);
+hipError_t launch_direct(const DirectLaunch& L, hipStream_t s);
 hipError_t launch_debug_philox(uint64_t seed, const uint4* ctr, uint4* out, uint32_t n, hipStream_t s);
 hipError_t launch_debug_fp64_math(int fn, const double* x, uint32_t n, double* out, hipStream_t s);
 hipError_t launch_debug_draws(const McDesc* desc, uint64_t pair0, uint64_t n_pairs, int fp32, double* za,

@@ -72,6 +72,28 @@ def test_surface_equals_single_calls_bitwise(ope, precision, n_paths):
             assert (prices[e, k], ses[e, k]) == (r.price, r.std_error), (t, K)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_direct_and_staged_launches_agree_bitwise(ope, precision):
+    # One request per launch group with at most 16 strikes runs as a direct
+    # launch (descriptor and strikes in the kernel arguments, the last block
+    # reducing the slots); 17 strikes take the staged upload and the separate
+    # reduction launch.  The 16 shared strikes agree bit for bit, and both
+    # equal single calls (direct, one strike each).
+    strikes = list(np.linspace(70.0, 130.0, 17))
+    cfg = ope.McConfig(n_paths=300_000, n_steps=40, seed=9, scheme=1)
+    hp = ope.HestonParams(**BASE)
+    p16, s16, _, _ = ope.mc_surface(100.0, 0.03, 0.0, hp, JUMP, 0, [0.75], strikes[:16], cfg, precision)
+    p17, s17, _, _ = ope.mc_surface(100.0, 0.03, 0.0, hp, JUMP, 0, [0.75], strikes, cfg, precision)
+    assert np.array_equal(p16[0], p17[0, :16]) and np.array_equal(s16[0], s17[0, :16])
+    for k in (0, 7, 15, 16):
+        r = ope.mc_price_ex(100.0, 0.75, 0.03, 0.0, hp, ope.Payoff.European(strikes[k]), cfg, JUMP,
+                            ope.McOptions(precision)).result
+        assert (p17[0, k], s17[0, k]) == (r.price, r.std_error), k
+    # repeated direct calls: each launch's completion counter starts from zero
+    again, _, _, _ = ope.mc_surface(100.0, 0.03, 0.0, hp, JUMP, 0, [0.75], strikes[:16], cfg, precision)
+    assert np.array_equal(again, p16)
+
+
 def _uneven_requests(ope, n, seed):
     rng = np.random.default_rng(seed)
     reqs = []
