@@ -174,8 +174,8 @@ int ope_mc_bates_ex(double spot, double expiry, double rate, double div_yield,
 /* No counterpart in the reference.  ope_mc_bates_ex for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): the call allocates nothing; its device buffers live in a
- * fixed 385 KiB array in the library's device code (one per GPU, held until
- * the process ends).  The first call on a device also initialises the
+ * fixed 256.25 KiB array in the library's device code (one per GPU, held
+ * until the process ends).  The first call on a device also initialises the
  * library's context and looks the array up.  Philox draws only (no replay,
  * no per-path outputs); at most 2^24 antithetic pairs (2^25 paths), larger
  * requests fail with OPE_ERR_INVALID_ARGUMENT.  The result equals

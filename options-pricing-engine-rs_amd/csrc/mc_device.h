@@ -1112,16 +1112,17 @@ __device__ __forceinline__ void wave_sum2(double& s, double& q) {
     }
 }
 
-// Sum of n_tiles partial-sum slots slots[base + t * stride], t in tile order:
-// thread i takes tiles i, i + kBlock, ... sequentially, then the xor butterfly
-// over the wave, then waves 0..3 in order.  Every reduction of a request's
-// slots goes through this one function, so the order (and the bits) never
-// depend on which kernel reduces.  The result is valid in thread 0.
+// Sum of the n_tiles partial-sum slots slots[base + t] of one (request,
+// strike), t in tile order: thread i takes tiles i, i + kBlock, ...
+// sequentially (coalesced across the block), then the xor butterfly over the
+// wave, then waves 0..3 in order.  Every reduction of a request's slots goes
+// through this one function, so the order (and the bits) never depend on
+// which kernel reduces.  The result is valid in thread 0.
 __device__ __forceinline__ double2 block_sum_slots(const double2* slots, uint64_t base, uint32_t n_tiles,
-                                                   uint32_t stride, double2* w) {
+                                                   double2* w) {
     double s = 0.0, q = 0.0;
     for (uint32_t t = threadIdx.x; t < n_tiles; t += kBlock) {
-        const double2 v = slots[base + (uint64_t)t * stride];
+        const double2 v = slots[base + t];
         s += v.x;
         q += v.y;
     }
@@ -1281,7 +1282,7 @@ __device__ __forceinline__ void price_tile(const McDesc& d, uint32_t tile, const
             if (tid < kEpiStrikes && k0 + tid < ns) {
                 const double2 a = wsum[tid * 4 + 0], b = wsum[tid * 4 + 1], e = wsum[tid * 4 + 2],
                               f = wsum[tid * 4 + 3];
-                slots[d.slot_off + (uint64_t)tile * ns + k0 + tid] =
+                slots[d.slot_off + (uint64_t)(k0 + tid) * d.n_tiles + tile] =
                     make_double2((((a.x + b.x) + e.x) + f.x) * d.disc, (((a.y + b.y) + e.y) + f.y) * disc2);
             }
         }
@@ -1310,7 +1311,7 @@ __device__ __forceinline__ void price_tile(const McDesc& d, uint32_t tile, const
         __syncthreads();
         for (uint32_t k = tid; k < ns; k += kBlock) {
             const double2 a = lds[k * 4 + 0], b = lds[k * 4 + 1], c = lds[k * 4 + 2], e = lds[k * 4 + 3];
-            slots[d.slot_off + (uint64_t)tile * ns + k] =
+            slots[d.slot_off + (uint64_t)k * d.n_tiles + tile] =
                 make_double2(((a.x + b.x) + c.x) + e.x, ((a.y + b.y) + c.y) + e.y);
         }
     }

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kBlock, (pinned_waves<Real, CODE>())) void k_tiles_
     if (!last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     for (uint32_t k = 0; k < a.d.n_strikes; ++k) {
-        const double2 r = block_sum_slots(slots, a.d.slot_off + k, a.d.n_tiles, a.d.n_strikes, w);
+        const double2 r = block_sum_slots(slots, a.d.slot_off + (uint64_t)k * a.d.n_tiles, a.d.n_tiles, w);
         if (threadIdx.x == 0) out[a.d.out_off + k] = r;
         __syncthreads();  // w is rewritten by the next strike's sum
     }

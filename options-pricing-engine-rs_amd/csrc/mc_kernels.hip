@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const McDesc* __restrict__ de
     __shared__ double2 w[kBlock / 64];
     const uint2 jb = jobs[blockIdx.x];
     const McDesc& d = descs[jb.x];
-    const double2 r = block_sum_slots(slots, d.slot_off + jb.y, d.n_tiles, d.n_strikes, w);
+    const double2 r = block_sum_slots(slots, d.slot_off + (uint64_t)jb.y * d.n_tiles, d.n_tiles, w);
     if (threadIdx.x == 0) out[d.out_off + jb.y] = r;
 }
 

@@ -2,19 +2,22 @@
 // that cannot call hipMalloc (code under HIP graph capture, tight serving
 // loops, harnesses that forbid it).
 //
-// Every device buffer of the call lives in `scratch`, a file-scope device
-// array of kScratchBytes: the request's descriptor, strike and reduction job,
-// the tile list, one (sum, sum_sq) slot per tile, and the reduced sums.  At
-// the first call on a device (with that device current) hipGetSymbolAddress
-// gives the array's device address, kept per device; host code passes only
-// that address to kernels and copies, never the array's name.
+// The call runs as one direct launch (mc_direct.hip): the descriptor and the
+// strike travel in the kernel arguments, and the device buffers it needs live
+// in `scratch`, a file-scope device array of kScratchBytes: the launch's
+// completion counter, the reduced sums and one (sum, sum_sq) slot per tile.
+// At the first call on a device (with that device current) hipGetSymbolAddress
+// gives the array's device address, kept per device, and a one-thread kernel
+// clears the counter; host code passes only that address to kernels and
+// copies, never the array's name.  After that, each launch's last block clears
+// the counter again; every slot is written by its tile before the reduction
+// reads it.
 //
-// The call plans, launches and reduces exactly as ope_mc_bates_ex does for
-// one request (same tiling, same specialised kernel, same tile-order
-// reduction), so its result is bit-for-bit that of ope_mc_bates_ex.  Nothing
-// in the array has to start at zero: every slot is written by its tile
-// before the reduction reads it.  Calls are serialised by the library's
-// context lock; the array keeps the last call's contents.
+// ope_mc_bates_ex runs a request of up to 16 strikes as the same direct
+// launch (same tiling, same kernel, same tile-order reduction), so the result
+// is bit for bit ope_mc_bates_ex's.  Calls are serialised by the library's
+// context lock (calls that overlapped on one GPU would race on the array); the
+// array keeps the last call's contents.
 //
 // Limit: at most kMaxTiles tiles, i.e. 2^24 antithetic pairs (2^25 paths).
 #include <hip/hip_runtime.h>
@@ -28,34 +31,28 @@ namespace ope {
 namespace {
 
 constexpr uint32_t kMaxTiles = 16384;
-constexpr size_t kHeaderBytes = 1024;  // descriptor, strike, job, reduced sums
-constexpr size_t kOffDesc = 0, kOffStrike = 512, kOffJob = 528, kOffOut = 544;
-constexpr size_t kOffTiles = kHeaderBytes;
-constexpr size_t kOffSlots = kOffTiles + (size_t)kMaxTiles * sizeof(uint2);
+constexpr size_t kHeaderBytes = 256;  // completion counter, reduced sums
+constexpr size_t kOffDone = 0, kOffOut = 16;
+constexpr size_t kOffSlots = kHeaderBytes;
 constexpr size_t kScratchBytes = kOffSlots + (size_t)kMaxTiles * sizeof(double2);
 constexpr int kMaxDevices = 64;
 
-static_assert(sizeof(McDesc) <= kOffStrike, "descriptor fits its header field");
-static_assert(kOffStrike + sizeof(double) <= kOffJob && kOffJob + sizeof(uint2) <= kOffOut, "header layout");
-static_assert(kOffOut + sizeof(double2) <= kHeaderBytes && kOffSlots % 256 == 0, "header layout");
+static_assert(kOffDone + sizeof(uint32_t) <= kOffOut && kOffOut + sizeof(double2) <= kHeaderBytes, "header layout");
+static_assert(kOffSlots % 256 == 0, "slot alignment");
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 // Device address of `scratch` on each device (null until that device's first call).
 char* g_scratch[kMaxDevices] = {};
 
-// The single request's tile list: tile t of request 0.
-__global__ void k_single_tiles(uint2* tiles, uint32_t n) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) tiles[t] = make_uint2(0u, t);
-}
+__global__ void k_clear_counter(uint32_t* done) { *done = 0u; }
 
 struct Call {
     McDesc d;
     double strike;
     int fp32;
     double2 sums;
-    float ms, kms;
+    float ms;
 };
 
 // Runs under the context lock (host::with_context).
@@ -65,51 +62,32 @@ int run(void* arg, const host::ContextView& v) {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return fail(OPE_ERR_NO_DEVICE, "device %d beyond the supported range", dev);
+    hipStream_t s = v.stream;
     if (!g_scratch[dev]) {  // set-up, once per device, with that device current
         void* p = nullptr;
         HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(1), 0, s,
+                           reinterpret_cast<uint32_t*>(static_cast<char*>(p) + kOffDone));
+        HIP_TRY(hipGetLastError());
         g_scratch[dev] = static_cast<char*>(p);
     }
     char* base = g_scratch[dev];
-    McDesc* d_desc = reinterpret_cast<McDesc*>(base + kOffDesc);
-    double* d_strike = reinterpret_cast<double*>(base + kOffStrike);
-    uint2* d_job = reinterpret_cast<uint2*>(base + kOffJob);
     double2* d_out = reinterpret_cast<double2*>(base + kOffOut);
-    uint2* d_tiles = reinterpret_cast<uint2*>(base + kOffTiles);
-    double2* d_slots = reinterpret_cast<double2*>(base + kOffSlots);
 
-    // descriptor, strike and job in one copy from the caller's stack
-    struct alignas(16) Header {
-        unsigned char bytes[kOffOut];
-    } h;
-    std::memset(&h, 0, sizeof h);
-    std::memcpy(h.bytes + kOffDesc, &c.d, sizeof(McDesc));
-    std::memcpy(h.bytes + kOffStrike, &c.strike, sizeof(double));
-    const uint2 job = make_uint2(0u, 0u);
-    std::memcpy(h.bytes + kOffJob, &job, sizeof job);
-    hipStream_t s = v.stream;
-    HIP_TRY(hipMemcpyAsync(base, h.bytes, sizeof h.bytes, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_single_tiles, dim3((c.d.n_tiles + 255) / 256), dim3(256), 0, s, d_tiles, c.d.n_tiles);
-    HIP_TRY(hipGetLastError());
-
-    TileLaunch L;
-    L.descs = d_desc;
-    L.tiles = d_tiles;
-    L.n_tiles = c.d.n_tiles;
-    L.strikes = d_strike;
-    L.slots = d_slots;
+    DirectLaunch L;
+    std::memset(&L, 0, sizeof L);
+    L.args.d = c.d;
+    L.args.strikes[0] = c.strike;
+    L.slots = reinterpret_cast<double2*>(base + kOffSlots);
+    L.done = reinterpret_cast<uint32_t*>(base + kOffDone);
+    L.out = d_out;
     L.fp32 = c.fp32;
-    L.fixed = 1;
-    L.code = config_code(c.d);
     HIP_TRY(hipEventRecord(v.ev0, s));
-    HIP_TRY(launch_tiles(L, s));
-    HIP_TRY(hipEventRecord(v.evk, s));
-    HIP_TRY(launch_reduce(d_desc, d_job, 1, d_slots, d_out, s));
+    HIP_TRY(launch_direct(L, s));
     HIP_TRY(hipEventRecord(v.ev1, s));
     HIP_TRY(hipMemcpyAsync(&c.sums, d_out, sizeof(double2), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     HIP_TRY(hipEventElapsedTime(&c.ms, v.ev0, v.ev1));
-    HIP_TRY(hipEventElapsedTime(&c.kms, v.ev0, v.evk));
     return OPE_OK;
 }
 
@@ -143,6 +121,6 @@ extern "C" int ope_mc_bates_noalloc(double spot, double expiry, double rate, dou
     if ((rc = with_context(run, &c))) return rc;
     finish(c.d, c.sums, out, opts ? opts->stats : nullptr);
     if (opts && opts->device_ms) *opts->device_ms = c.ms;
-    if (opts && opts->kernel_ms) *opts->kernel_ms = c.kms;
+    if (opts && opts->kernel_ms) *opts->kernel_ms = c.ms;
     return OPE_OK;
 }

@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinned
         __syncthreads();
         if (s_last) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const double2 r = block_sum_slots(L.slots, rec.d.slot_off, it.n_tiles, 1, s_w);
+            const double2 r = block_sum_slots(L.slots, rec.d.slot_off, it.n_tiles, s_w);
             if (threadIdx.x == 0) {
                 const uint64_t idx =
                     __hip_atomic_fetch_add(&L.dc->comp_head, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
