@@ -31,14 +31,17 @@
 //               fma remainder; within 8 ulp (7e-16 relative, mostly the
 //               roundings of t and of the exponential).
 //   sincos_rev23  sin and cos of 2 pi k 2^-23 for a 23-bit integer k: the
-//               nearest quarter turn is taken in integer arithmetic (exact),
-//               the remaining angle (at most pi/4) goes through Taylor
-//               polynomials to degree 17 (sine) and 16 (cosine), and the
-//               quarter turn rotates the pair.
+//               nearest 1/256 turn is taken in integer arithmetic (exact)
+//               and looked up (gen_fp64_tables.py), the remaining angle (at
+//               most pi/256) goes through Taylor polynomials to degree 7
+//               (sine) and 6 (cosine), and the angle-sum formulas combine
+//               the two.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "fp64_tables.h"
 
 namespace ope {
 namespace {
@@ -151,36 +154,23 @@ __device__ __forceinline__ double ncdf_neg_abs_f64(double z) {
     return (t * g) * fma(ex, elo, ex);
 }
 
-// sin and cos of 2 pi k 2^-23, k in [0, 2^23).
+// sin and cos of 2 pi k 2^-23, k in [0, 2^23): the nearest 1/256 turn j
+// (integer arithmetic, exact) from kSinCos256, the rest b = 2 pi l 2^-23,
+// |b| <= pi/256, through sin b (degree 7) and cos b - 1 (degree 6), and the
+// angle-sum formulas with the cosine's deviation from 1 kept separate, so
+// that a zero of the table stays a zero (exact zeros of the sine at half
+// turns and of the cosine at odd quarter turns).
 __device__ __forceinline__ void sincos_rev23(uint32_t k, double& sn, double& cs) {
-    const uint32_t j = (k + (1u << 20)) >> 21;            // nearest quarter turn, 0..4
-    const int32_t dk = (int32_t)k - (int32_t)(j << 21);    // [-2^20, 2^20)
-    const double t = (double)dk * 7.490140565847857e-07;   // pi 2^-22: |t| <= pi/4
-    const double t2 = t * t;
-    double ps = 2.8114572543455206e-15;                    // 1/17!
-    ps = fma(ps, t2, -7.647163731819816e-13);
-    ps = fma(ps, t2, 1.6059043836821613e-10);
-    ps = fma(ps, t2, -2.505210838544172e-08);
-    ps = fma(ps, t2, 2.7557319223985893e-06);
-    ps = fma(ps, t2, -0.0001984126984126984);
-    ps = fma(ps, t2, 0.008333333333333333);
-    ps = fma(ps, t2, -0.16666666666666666);
-    const double s = fma(t * t2, ps, t);
-    double pc = 4.779477332387385e-14;                     // 1/16!
-    pc = fma(pc, t2, -1.1470745597729725e-11);
-    pc = fma(pc, t2, 2.08767569878681e-09);
-    pc = fma(pc, t2, -2.755731922398589e-07);
-    pc = fma(pc, t2, 2.48015873015873e-05);
-    pc = fma(pc, t2, -0.001388888888888889);
-    pc = fma(pc, t2, 0.041666666666666664);
-    pc = fma(pc, t2, -0.5);
-    const double c = fma(t2, pc, 1.0);
-    // rotate by j quarter turns: (s, c) -> (c, -s) -> (-s, -c) -> (-c, s)
-    const uint32_t q = j & 3u;
-    const double a = (q & 1u) ? c : s;
-    const double b = (q & 1u) ? s : c;
-    sn = (q & 2u) ? -a : a;
-    cs = (q == 1u || q == 2u) ? -b : b;
+    const uint32_t j = (k + (1u << 14)) >> 15;              // 0..256
+    const int32_t l = (int32_t)k - (int32_t)(j << 15);      // [-2^14, 2^14)
+    const double2 t = kSinCos256[j & 255u];                 // (sin, cos) of 2 pi j / 256
+    const double b = (double)l * 7.4901405658478570e-07;    // 2 pi 2^-23
+    const double b2 = b * b;
+    const double sb = fma(b * b2, fma(b2, fma(b2, -1.9841269841269841e-04, 8.3333333333333333e-03),
+                                      -1.6666666666666667e-01), b);
+    const double cm = b2 * fma(b2, fma(b2, -1.3888888888888889e-03, 4.1666666666666667e-02), -0.5);
+    sn = fma(t.y, sb, fma(t.x, cm, t.x));
+    cs = fma(-t.x, sb, fma(t.y, cm, t.y));
 }
 
 }  // namespace
