@@ -16,13 +16,18 @@
 //               two corrections g += (x - g^2) h.  No scaling for denormal or
 //               infinite input; x = 0 gives 0 (the rsq input is floored at
 //               the least normal, so g = 0 y0 = 0).
-//   log_f64     x = 2^k m with m in [sqrt(1/2), sqrt 2) (frexp, exact);
-//               f = m - 1 exactly (Sterbenz); with s = f / (2 + f),
-//               log(1 + f) = f - (f^2/2 - s (f^2/2 + R)), R = sum 2 s^2j/(2j+1)
-//               (the atanh series; the decomposition of fdlibm's e_log.c),
-//               and k ln2 added as k ln2_hi + k ln2_lo (Cody and Waite).
-//   exp_f64     Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, Taylor
-//               polynomial of e^r to degree 14, scaled by 2^k (ldexp).
+//   log_f64     x = 2^k m with m in [sqrt(1/2), sqrt 2) (frexp, exact); m
+//               rounded to i/128 picks a table entry (gen_fp64_tables.py)
+//               with inv ~ 128/i and -ln(inv) in two parts; ln m =
+//               ln1p(r) - ln(inv) with r = m inv - 1 (one fma, |r| <= 0.0056,
+//               exact where inv = 1, so ln(1 + eps) keeps its relative
+//               accuracy), ln1p(r) = r + r^2 q(r) with the Taylor series to
+//               degree 7, and k ln2 added as k ln2_hi + k ln2_lo (Cody and
+//               Waite; Tang's table-driven method).
+//   exp_f64     x = (64 e + j) ln2/64 + r with k = 64 e + j = rint(64 x/ln2),
+//               the reduction in two parts (k ln2/64_hi exact), e^r - 1 to
+//               degree 6 (|r| <= ln2/128), times 2^(j/64) from a table in
+//               two parts, scaled by 2^e (ldexp; Tang's method).
 //   ncdf_neg_abs_f64  Phi(-|z|) for |z| <= 6.8 (every Box-Muller normal of
 //               the engine: u1 >= 2^-33): exp(-z^2/2) t g(t) with t = 1/(1 +
 //               |z|/5) and g a degree-16 polynomial in the affine image of t
@@ -48,7 +53,6 @@ namespace {
 
 constexpr double kLn2Hi = 0.6931471806019545;       // ln 2 to 29 significant bits: k ln2_hi is exact
 constexpr double kLn2Lo = -4.2009150726810846e-11;  // ln 2 - ln2_hi
-constexpr double kLog2eD = 1.4426950408889634;
 
 __device__ __forceinline__ double rcp_f64(double x) {
     double y = __builtin_amdgcn_rcp(x);
@@ -85,45 +89,34 @@ __device__ __forceinline__ double log_f64(double x) {
     const bool lo = m < 0.70710678118654752;
     m = lo ? m + m : m;  // [sqrt(1/2), sqrt 2)
     k = lo ? k - 1 : k;
-    const double f = m - 1.0;
-    const double s = f * rcp_f64(2.0 + f);
-    const double z = s * s;
-    double R = 0.09523809523809523;  // 2/21
-    R = fma(R, z, 0.10526315789473684);
-    R = fma(R, z, 0.11764705882352941);
-    R = fma(R, z, 0.13333333333333333);
-    R = fma(R, z, 0.15384615384615385);
-    R = fma(R, z, 0.18181818181818182);
-    R = fma(R, z, 0.2222222222222222);
-    R = fma(R, z, 0.2857142857142857);
-    R = fma(R, z, 0.4);
-    R = fma(R, z, 0.6666666666666666);  // 2/3
-    R = R * z;
-    const double hfsq = 0.5 * f * f;
+    const int i = (int)__builtin_rint(m * 128.0);  // 91..181
+    const LogEntry t = kLogTab[i - 91];
+    const double r = fma(m, t.inv, -1.0);  // |r| <= 0.0056; exact at i = 128
+    double q = 0.14285714285714285;        // ln1p(r) = r + r^2 q(r), q to degree 5
+    q = fma(q, r, -0.16666666666666666);
+    q = fma(q, r, 0.2);
+    q = fma(q, r, -0.25);
+    q = fma(q, r, 0.33333333333333331);
+    q = fma(q, r, -0.5);
     const double kd = (double)k;
-    return fma(kd, kLn2Hi, f - (hfsq - fma(s, hfsq + R, kd * kLn2Lo)));
+    const double tail = fma(r * r, q, fma(kd, kLn2Lo, t.llo));
+    return fma(kd, kLn2Hi, t.lhi + (r + tail));
 }
 
-// e^x for |x| < 700.
+// e^x for |x| < 700: x = (64 e + j) ln2/64 + r, |r| <= ln2/128, e^x =
+// 2^e 2^(j/64) e^r with e^r - 1 to degree 6.
 __device__ __forceinline__ double exp_f64(double x) {
-    const double kd = __builtin_rint(x * kLog2eD);
-    const double r = fma(-kd, kLn2Lo, fma(-kd, kLn2Hi, x));
-    double p = 1.1470745597729725e-11;  // 1/14!
-    p = fma(p, r, 1.6059043836821613e-10);
-    p = fma(p, r, 2.08767569878681e-09);
-    p = fma(p, r, 2.505210838544172e-08);
-    p = fma(p, r, 2.755731922398589e-07);
-    p = fma(p, r, 2.7557319223985893e-06);
-    p = fma(p, r, 2.48015873015873e-05);
-    p = fma(p, r, 0.0001984126984126984);
-    p = fma(p, r, 0.001388888888888889);
-    p = fma(p, r, 0.008333333333333333);
-    p = fma(p, r, 0.041666666666666664);
-    p = fma(p, r, 0.16666666666666666);
+    const double kd = __builtin_rint(x * 92.332482616893658);  // 64 / ln 2
+    const int k = (int)kd;
+    const double r = fma(-kd, kLn2By64Lo, fma(-kd, kLn2By64Hi, x));
+    double p = 1.3888888888888889e-03;  // 1/720
+    p = fma(p, r, 8.3333333333333333e-03);
+    p = fma(p, r, 4.1666666666666667e-02);
+    p = fma(p, r, 1.6666666666666667e-01);
     p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
-    return __builtin_amdgcn_ldexp(p, (int)kd);
+    p = fma(r * r, p, r);  // e^r - 1
+    const double2 t = kExp2Tab[k & 63];
+    return __builtin_amdgcn_ldexp(t.x + fma(t.x, p, t.y), k >> 6);
 }
 
 // Phi(-|z|) = erfc(|z|/sqrt 2)/2 for |z| <= 6.8.

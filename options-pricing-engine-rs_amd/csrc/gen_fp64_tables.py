@@ -6,6 +6,13 @@ rounded once to double.
 
 * kSinCos256[j] = (sin, cos) of 2 pi j / 256, j = 0..255 (zeros of sine and
   cosine exactly 0).
+* kLogTab[i - 91], i = 91..181 (the mantissa m in [sqrt(1/2), sqrt 2)
+  rounded to i/128): (inv, lhi, llo) with inv = 1/(i/128) rounded once and
+  lhi + llo = -ln(inv) of that rounded inv (double-double), so that
+  ln m = ln1p(m inv - 1) + lhi + llo holds exactly; i = 128 is (1, 0, 0).
+* kExp2Tab[j] = (hi, lo) of 2^(j/64), j = 0..63, and the split of ln2/64
+  into kLn2By64Hi (32 significant bits: k kLn2By64Hi is exact for |k| <
+  2^17) and kLn2By64Lo.
 """
 import os
 
@@ -33,6 +40,33 @@ def main():
         if j % 128 == 64:
             c = mp.mpf(0)
         lines.append(f"    {{{d(s)}, {d(c)}}},")
+    lines.append("};")
+    lines.append("")
+    lines.append("// ln m = ln1p(m inv - 1) + lhi + llo for m rounded to i / 128, i = 91..181.")
+    lines.append("struct LogEntry {")
+    lines.append("    double inv, lhi, llo, pad;")
+    lines.append("};")
+    lines.append("__device__ const LogEntry kLogTab[91] = {")
+    for i in range(91, 182):
+        inv = mp.mpf(float(mp.mpf(128) / i))  # 1/c rounded once to double
+        L = -mp.log(inv)
+        hi = mp.mpf(float(L))
+        lo = L - hi
+        lines.append(f"    {{{d(inv)}, {d(hi)}, {d(lo)}, 0.0}},")
+    lines.append("};")
+    lines.append("")
+    ln2_64 = mp.log(2) / 64
+    e = int(mp.floor(mp.log(ln2_64, 2)))
+    hi = mp.floor(ln2_64 / mp.mpf(2) ** (e - 31)) * mp.mpf(2) ** (e - 31)  # 32 significant bits
+    lo = ln2_64 - hi
+    lines.append(f"constexpr double kLn2By64Hi = {d(hi)};")
+    lines.append(f"constexpr double kLn2By64Lo = {d(lo)};")
+    lines.append("// (hi, lo) of 2^(j/64), j = 0..63.")
+    lines.append("__device__ const double2 kExp2Tab[64] = {")
+    for j in range(64):
+        t = mp.mpf(2) ** (mp.mpf(j) / 64)
+        th = mp.mpf(float(t))
+        lines.append(f"    {{{d(th)}, {d(t - th)}}},")
     lines.append("};")
     lines += ["", "}  // namespace", "}  // namespace ope", ""]
     with open(os.path.join(HERE, "fp64_tables.h"), "w") as f:

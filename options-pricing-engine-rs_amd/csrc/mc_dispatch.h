@@ -30,10 +30,18 @@ struct Code {
 // Waves per SIMD pinned with __launch_bounds__ for the FP32-path up-and-out
 // bodies: their cold FP64 re-simulation near the barrier must not raise the
 // register budget of the FP32 loops, so each keeps the occupancy it has
-// without that code (the rare FP64 pass spills instead).  1: no constraint.
+// without that code (the rare FP64 pass spills instead).  FP64: the QE
+// bodies at 3 waves (unpinned the compiler spends 169-177 VGPRs where 129-163
+// suffice without spilling, and the QE pair runs 3.63 ms at 3 waves against
+// 4.04 ms at 2) and the Bates Euler bodies at 5 (16.70 against 16.80 ms for
+// the surface).  1: no constraint.
 template <class Real, int CODE>
 constexpr int pinned_waves() {
-    if (!std::is_same<Real, float>::value) return 1;
+    if (!std::is_same<Real, float>::value) {
+        if (CODE == 18) return 4;
+        if (CODE >= 12) return 3;
+        return (CODE == 6 || CODE == 8 || CODE == 10) ? 5 : 1;
+    }
     switch (CODE) {
     case 4: return 8;
     case 5: return 4;
