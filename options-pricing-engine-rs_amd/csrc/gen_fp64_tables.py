@@ -26,7 +26,8 @@ def d(x):
     return float.hex(float(mp.mpf(x)))
 
 
-def main():
+def render():
+    """The header's text."""
     lines = ["// Generated by gen_fp64_tables.py (mpmath, 50 digits, each entry rounded once",
              "// to double).  Do not edit.",
              "#pragma once", "", "#include <hip/hip_runtime.h>", "", "namespace ope {", "namespace {", ""]
@@ -69,8 +70,12 @@ def main():
         lines.append(f"    {{{d(th)}, {d(t - th)}}},")
     lines.append("};")
     lines += ["", "}  // namespace", "}  // namespace ope", ""]
+    return "\n".join(lines)
+
+
+def main():
     with open(os.path.join(HERE, "fp64_tables.h"), "w") as f:
-        f.write("\n".join(lines))
+        f.write(render())
 
 
 if __name__ == "__main__":

@@ -137,3 +137,54 @@ def test_no_gpu_is_an_error_not_a_fallback(ope):
     with pytest.raises(ope.OpeError) as e:
         ope.mc_heston(100.0, 1.0, 0.03, 0.0, _hp(ope), ope.Payoff.European(100.0), ope.McConfig(n_paths=100))
     assert e.value.status == ope.OPE_ERR_NO_DEVICE
+
+
+def test_fp64_tables_are_the_generator_output():
+    # csrc/fp64_tables.h (the FP64 path's sine / cosine, logarithm and
+    # exponential tables) is what gen_fp64_tables.py writes from mpmath
+    import importlib.util
+    import os
+
+    csrc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "options-pricing-engine-rs_amd", "csrc")
+    spec = importlib.util.spec_from_file_location("gen_fp64_tables", os.path.join(csrc, "gen_fp64_tables.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    with open(os.path.join(csrc, "fp64_tables.h")) as f:
+        assert f.read() == gen.render()
+
+
+def test_fp64_table_entries_are_exact_where_the_functions_are():
+    # the entries that must be exact: sin(pi) = cos(pi/2) = 0, the logarithm
+    # table's entry at 1, 2^0; and every other entry within half an ulp
+    import mpmath as mp
+
+    import importlib.util
+    import os
+
+    csrc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "options-pricing-engine-rs_amd", "csrc")
+    text = open(os.path.join(csrc, "fp64_tables.h")).read()
+
+    def rows(name):
+        body = text.split(name, 1)[1].split("};", 1)[0]
+        return [[float.fromhex(x.strip()) for x in r.split("}")[0].split(",") if x.strip()]
+                for r in body.split("{")[2:]]
+
+    mp.mp.dps = 40
+    sc = [r[:2] for r in rows("kSinCos256[256]")]
+    assert len(sc) == 256 and sc[0] == [0.0, 1.0] and sc[64] == [1.0, 0.0] and sc[128] == [0.0, -1.0]
+    for j in range(256):
+        a = 2 * mp.pi * j / 256
+        for got, want in zip(sc[j], (mp.sin(a), mp.cos(a))):
+            if want != 0 and abs(want) > 1e-30:
+                assert abs(mp.mpf(got) - want) <= mp.mpf(2) ** (mp.floor(mp.log(abs(want), 2)) - 53)
+    logs = rows("kLogTab[91]")
+    assert len(logs) == 91 and logs[128 - 91][:3] == [1.0, 0.0, 0.0]
+    for i, (inv, hi, lo, _) in enumerate(logs):
+        assert abs(inv - 128.0 / (i + 91)) <= 2.0 ** -52 * 2
+        assert abs(mp.mpf(hi) + mp.mpf(lo) + mp.log(mp.mpf(inv))) < mp.mpf(10) ** -30
+    ex = rows("kExp2Tab[64]")
+    assert len(ex) == 64 and ex[0] == [1.0, 0.0]
+    for j, (hi, lo) in enumerate(ex):
+        assert abs(mp.mpf(hi) + mp.mpf(lo) - mp.mpf(2) ** (mp.mpf(j) / 64)) < mp.mpf(10) ** -30
