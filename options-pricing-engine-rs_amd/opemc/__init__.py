@@ -101,6 +101,32 @@ class _Request(C.Structure):
                 ("cfg", _Config)]
 
 
+def _flat_dtype(struct) -> np.dtype:
+    """A numpy dtype with the byte layout of a ctypes Structure, nested
+    structures flattened to dotted names and byte arrays (reserved fields)
+    left as zero padding, so a batch of structures fills from Python tuples
+    in one assignment."""
+    names, formats, offsets = [], [], []
+
+    def walk(st, base, prefix):
+        for name, ct, *_ in st._fields_:
+            off = base + getattr(st, name).offset
+            if issubclass(ct, C.Structure):
+                walk(ct, off, prefix + name + ".")
+            elif not issubclass(ct, C.Array):
+                names.append(prefix + name)
+                formats.append(np.dtype(ct))
+                offsets.append(off)
+
+    walk(struct, 0, "")
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": C.sizeof(struct)})
+
+
+_REQUEST_NP = _flat_dtype(_Request)
+_RESULT_NP = _flat_dtype(_Result)
+_STATS_NP = _flat_dtype(_Stats)
+
+
 class _DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int32),
                 ("clock_mhz", C.c_int32), ("total_mem_bytes", C.c_uint64)]
@@ -305,7 +331,9 @@ class _OptionsHolder:
                 self.c.path_states = self.path_states.ctypes.data_as(C.POINTER(C.c_double))
 
     def stats_list(self):
-        return [McStats(s.sum, s.sum_sq, s.count, s.path_steps) for s in self.stats]
+        v = np.frombuffer(self.stats, _STATS_NP)
+        return [McStats(*s) for s in zip(v["sum"].tolist(), v["sum_sq"].tolist(), v["count"].tolist(),
+                                         v["path_steps"].tolist())]
 
 
 def mc_heston(spot, expiry, rate, div_yield, params: HestonParams, payoff: Payoff, cfg: McConfig) -> McResult:
@@ -371,8 +399,9 @@ def mc_surface(spot, rate, div_yield, params: HestonParams, jumps, opt_type, exp
                                 ex.ctypes.data_as(C.POINTER(C.c_double)), ex.size,
                                 ks.ctypes.data_as(C.POINTER(C.c_double)), ks.size, C.byref(cfg._c()),
                                 C.byref(h.c), out))
-    prices = np.array([r.price for r in out]).reshape(ex.size, ks.size)
-    ses = np.array([r.std_error for r in out]).reshape(ex.size, ks.size)
+    v = np.frombuffer(out, _RESULT_NP)
+    prices = v["price"].reshape(ex.size, ks.size).copy()
+    ses = v["std_error"].reshape(ex.size, ks.size).copy()
     if with_kernel_ms:
         return prices, ses, h.stats_list(), h.ms.value, h.kms.value
     return prices, ses, h.stats_list(), h.ms.value
@@ -401,16 +430,25 @@ class McRequest:
         r.cfg = self.cfg._c()
         return r
 
+    def _row(self):
+        """The fields of ope_mc_request in its order (a row of _REQUEST_NP)."""
+        p, q, c = self.params, self.payoff, self.cfg
+        return (self.spot, self.expiry, self.rate, self.div_yield, p.v0, p.kappa, p.theta, p.sigma, p.rho,
+                self.lambda_, self.mu_j, self.sigma_j, q.kind, int(q.opt_type), q.strike, q.barrier, q.rebate,
+                c.n_paths, c.n_steps, c.seed, 1 if c.antithetic else 0, int(c.scheme))
+
 
 def mc_batch(requests: Sequence[McRequest], precision=PRECISION_FP64, with_kernel_ms=False):
     """All requests in one kernel launch.  Returns (results, stats, device_ms),
     plus the path-kernel time when with_kernel_ms."""
     n = len(requests)
-    arr = (_Request * n)(*[r._c() for r in requests])
+    arr = np.zeros(max(n, 1), _REQUEST_NP)
+    arr[:n] = [r._row() for r in requests]
     h = _OptionsHolder(McOptions(precision), max(n, 1))
     out = (_Result * max(n, 1))()
-    _check(lib().ope_mc_batch(arr, n, C.byref(h.c), out))
-    res = [McResult(out[i].price, out[i].std_error) for i in range(n)]
+    _check(lib().ope_mc_batch(arr.ctypes.data_as(C.POINTER(_Request)), n, C.byref(h.c), out))
+    v = np.frombuffer(out, _RESULT_NP)[:n]
+    res = [McResult(p, s) for p, s in zip(v["price"].tolist(), v["std_error"].tolist())]
     if with_kernel_ms:
         return res, h.stats_list()[:n], h.ms.value, h.kms.value
     return res, h.stats_list()[:n], h.ms.value

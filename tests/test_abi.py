@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -66,6 +67,26 @@ def test_struct_layout(ope):
     assert ope._ServerConfig.ring_items.offset == 20 and ope._ServerConfig.idle_exit_ms.offset == 24
     assert ope._ServerConfig.watchdog_ms.offset == 32
     assert C.sizeof(ope._ServerRequestInfo) == 48 and ope._ServerRequestInfo.stop_reason.offset == 40
+
+
+def test_batch_rows_have_the_request_struct_bytes(ope):
+    """mc_batch fills its request array from numpy rows; each row's bytes
+    equal the ctypes structure the single-request calls pass."""
+    H = ope.HestonParams(0.04, 1.5, 0.05, 0.5, -0.7)
+    reqs = [
+        ope.McRequest(100.0, 1.0, 0.03, 0.01, H, ope.Payoff(0, 95.0), ope.McConfig(n_paths=10_000, n_steps=252)),
+        ope.McRequest(80.0, 0.25, -0.01, 0.0, H, ope.Payoff(2, 90.0, ope.OptionType.Put, 120.0, 1.5),
+                      ope.McConfig(n_paths=3, n_steps=1, seed=2**64 - 1, antithetic=False,
+                                   scheme=ope.VarianceScheme.QuadraticExponential), 0.7, -0.1, 0.15),
+        ope.McRequest(1e-3, 30.0, 0.2, 0.1, H, ope.Payoff(1, 1e6, ope.OptionType.Call),
+                      ope.McConfig(n_paths=2**40, n_steps=100_000, seed=0)),
+    ]
+    rows = np.zeros(len(reqs), ope._REQUEST_NP)
+    rows[:] = [r._row() for r in reqs]
+    assert rows.itemsize == C.sizeof(ope._Request)
+    assert rows.tobytes() == bytes((ope._Request * len(reqs))(*[r._c() for r in reqs]))
+    res = (ope._Result * 2)((1.25, 0.5), (-3.0, 7.0))
+    assert np.frombuffer(res, ope._RESULT_NP).tolist() == [(1.25, 0.5), (-3.0, 7.0)]
 
 
 def test_config_default_is_reference_default(ope):
