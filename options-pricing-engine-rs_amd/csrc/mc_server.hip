@@ -52,8 +52,14 @@ struct SrvMirror {
     uint32_t pad_;
 };
 
+// Polling loads are relaxed, and the acquire is one fence taken after the
+// awaited value has arrived.  An acquire load invalidates the L2 each time it
+// executes (buffer_inv); with a thousand idle workgroups polling, those
+// invalidations queued behind each other and every memory access of the
+// running tiles waited ~30 us (a one-step request took 0.14 ms end to end,
+// 0.02 ms with relaxed polls).
 __device__ __forceinline__ uint64_t sys_load64(const uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ uint32_t sys_load32(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -69,6 +75,9 @@ __device__ void sentinel(const SrvLaunch& L, SrvMirror* mirror) {
         const uint32_t stop = sys_load32(&L.hc->shutdown);
         const uint64_t tk = __hip_atomic_load(&mirror->tickets, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (t != last) {
+            // pairs with the host's release store of the tail: the items below
+            // it are visible to the workers that acquire the mirrored tail
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
             __hip_atomic_store(&mirror->tail, t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             last = t;
             mirrored = t;
@@ -112,8 +121,9 @@ __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinned
             uint32_t status = 0;
             SrvItem it{0, 0, 0, 0};
             for (;;) {
-                const uint64_t t = __hip_atomic_load(&mirror->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t t = __hip_atomic_load(&mirror->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (ticket < t) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     const uint64_t* src = reinterpret_cast<const uint64_t*>(L.ring + (ticket % L.ring_cap));
                     const uint64_t w0 = sys_load64(src), w1 = sys_load64(src + 1);
                     it.slot = (uint32_t)w0;
@@ -122,7 +132,8 @@ __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinned
                     it.n_tiles = (uint32_t)(w1 >> 32);
                     break;
                 }
-                if (__hip_atomic_load(&mirror->shutdown, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+                if (__hip_atomic_load(&mirror->shutdown, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     // the sentinel stored its final tail before the flag
                     if (ticket >= __hip_atomic_load(&mirror->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                         status = 1;
