@@ -14,8 +14,11 @@
  *   - SmallRng::seed_from_u64 in rand 0.8 does not forward to the generator's
  *     own SplitMix64 seeding; it falls through to rand_core 0.6's default,
  *     which fills the 32-byte seed with PCG32 (XSH-RR) outputs, advancing the
- *     LCG state before each output.  ASSUMPTION: only the crate could confirm
- *     it, so crate-identical draws are unverified.
+ *     LCG state before each output.  Pinned by rand_core 0.6's own
+ *     value-breakage answer (seed 0 expands to first word
+ *     5029875928683246316, tests/test_oracle_rng.py); the uniform maps and
+ *     draw_step's word order below are pinned there too, so crate-identical
+ *     draws are open only up to the two last-bit questions that follow.
  *   - gen::<f64>() = (next_u64 >> 11) * 2^-53.
  *   - gen_range(lo..hi) for f64 = UniformFloat::sample_single: build a value in
  *     [1,2) from the top 52 bits, subtract 1, scale, add lo; retry if >= hi.
