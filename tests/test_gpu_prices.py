@@ -286,3 +286,34 @@ def test_one_step_asian_is_european_pathwise(ope, precision, scheme):
     else:
         # each S_T within a few FP32 roundings (1e-7 relative) of the other
         assert abs(a.price - e.price) <= 1e-5 * e.price and abs(a.std_error - e.std_error) <= 1e-4 * e.std_error
+
+
+EDGE = [
+    # (id, Heston overrides, expiry, rate, dividend yield, scheme, jumps)
+    ("rho_minus_1", dict(rho=-1.0), 1.0, 0.03, 0.01, 0, None),
+    ("rho_minus_1_qe_bates", dict(rho=-1.0), 1.0, 0.03, 0.01, 1, (0.5, -0.1, 0.15)),
+    ("rho_plus_1_qe", dict(rho=1.0), 1.0, 0.03, 0.01, 1, None),
+    ("rho_plus_1_bates", dict(rho=1.0), 1.0, 0.03, 0.01, 0, (0.5, -0.1, 0.15)),
+    ("rho_zero_qe", dict(rho=0.0), 1.0, 0.03, 0.0, 1, None),
+    ("negative_rate", {}, 2.0, -0.01, 0.05, 0, (0.5, -0.1, 0.15)),
+    ("short_expiry_qe", {}, 1e-3, 0.03, 0.0, 1, (0.5, -0.1, 0.15)),
+    ("fast_mean_reversion_qe", dict(kappa=50.0), 1.0, 0.03, 0.0, 1, None),
+]
+
+
+@pytest.mark.parametrize("case", EDGE, ids=[c[0] for c in EDGE])
+def test_fp32_tracks_fp64_at_edge_parameters(ope, case):
+    # the FP32 path mixes rho into the Box-Muller cosine through acos(rho)
+    # (0 and pi/2 .. pi at the ends); |rho| = 1, rho = 0, a negative rate, a
+    # 1e-3-year expiry and kappa dt = 12.5 stay within 0.1 SE of the FP64
+    # kernel on the same Philox draws, for every payoff
+    _, over, expiry, rate, q, scheme, jumps = case
+    hp = dict(BASE, **over)
+    cfg = ope.McConfig(n_paths=200_000, n_steps=3 if expiry < 0.01 else 24, seed=19, scheme=scheme)
+    for pay in (ope.Payoff.European(100.0), ope.Payoff.European(95.0, ope.OptionType.Put),
+                ope.Payoff.AsianArithmetic(100.0), ope.Payoff.UpAndOut(100.0, 130.0, 1.0)):
+        r64 = run(ope, 0, pay, cfg, hp=hp, jumps=jumps, expiry=expiry, rate=rate, q=q).result
+        r32 = run(ope, 1, pay, cfg, hp=hp, jumps=jumps, expiry=expiry, rate=rate, q=q).result
+        assert math.isfinite(r32.price) and math.isfinite(r32.std_error), (r32, r64)
+        # (the 95 put at the 1e-3-year expiry is ~8 sd out of the money: both 0)
+        assert abs(r32.price - r64.price) <= 0.1 * r64.std_error + 1e-12, (case[0], pay, r32, r64)

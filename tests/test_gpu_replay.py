@@ -22,19 +22,20 @@ JUMP = (0.5, -0.1, 0.15)
 
 
 def _oracle_and_gpu(ope, ora, scheme, lam, kind, anti, opt, n_pairs, n_steps, seed=42, precision=0, strike=100.0,
-                    barrier=130.0, rebate=0.0, jump_params=JUMP, hp=HP):
+                    barrier=130.0, rebate=0.0, jump_params=JUMP, hp=HP, expiry=T, rate=R, q=Q):
     jp = ora.jumps(lam, *jump_params[1:]) if lam is not None else ora.jumps()
-    za, zb, js = ora.gen_draws(seed, n_pairs, n_steps, T, jp, n_threads=4)
+    za, zb, js = ora.gen_draws(seed, n_pairs, n_steps, expiry, jp, n_threads=4)
     opay = ora.payoff(kind, strike, opt, barrier, rebate)
-    pay_ref, st_ref = ora.run_paths(S0, T, R, Q, ora.heston(**hp), opay, n_steps, scheme, anti, za, zb, js, jp)
+    pay_ref, st_ref = ora.run_paths(S0, expiry, rate, q, ora.heston(**hp), opay, n_steps, scheme, anti, za, zb, js,
+                                    jp)
     params = ope.HestonParams(**hp)
     payoff = ope.Payoff(kind, strike, opt, barrier, rebate)
     cfg = ope.McConfig(n_paths=n_pairs * (2 if anti else 1), n_steps=n_steps, seed=seed, antithetic=anti,
                        scheme=scheme)
     jumps = (lam, jump_params[1], jump_params[2]) if lam is not None else None
     print(f"replay case scheme={scheme} lam={lam} kind={kind} anti={anti} opt={opt} pairs={n_pairs} "
-          f"steps={n_steps} precision={precision}", flush=True)
-    run = ope.mc_price_ex(S0, T, R, Q, params, payoff, cfg, jumps,
+          f"steps={n_steps} precision={precision} hp={hp} T={expiry} r={rate} q={q}", flush=True)
+    run = ope.mc_price_ex(S0, expiry, rate, q, params, payoff, cfg, jumps,
                           ope.McOptions(precision, replay=(za, zb, js if lam else None), want_paths=True))
     return pay_ref, st_ref, run
 
@@ -153,3 +154,40 @@ def test_replay_negative_v0(ope, ora, scheme, lam, kind, n_steps, precision):
         np.testing.assert_allclose(st[..., 0], st_ref[..., 0], rtol=0, atol=2e-5)
         np.testing.assert_allclose(st[..., 1], st_ref[..., 1], rtol=0, atol=2e-5)
         assert (np.abs(run.path_payoffs - pay_ref) <= 5e-3).mean() > 0.998
+
+
+EDGE_PARAMS = [
+    # (id, Heston overrides, expiry, rate, dividend yield, scheme, lambda, payoff kind, steps)
+    ("rho_minus_1_euler", dict(rho=-1.0), 1.0, 0.03, 0.01, 0, None, 0, 24),
+    ("rho_minus_1_qe_bates_asian", dict(rho=-1.0), 1.0, 0.03, 0.01, 1, 0.5, 1, 24),
+    ("rho_plus_1_qe_uao", dict(rho=1.0), 1.0, 0.03, 0.01, 1, None, 2, 24),
+    ("rho_plus_1_euler_bates_uao", dict(rho=1.0), 1.0, 0.03, 0.01, 0, 0.5, 2, 24),
+    ("negative_rate_dividend_above_rate", {}, 2.0, -0.01, 0.05, 0, 0.5, 2, 30),
+    ("short_expiry_qe_bates", {}, 1e-3, 0.03, 0.0, 1, 0.5, 0, 3),
+    ("fast_mean_reversion_qe", dict(kappa=50.0), 1.0, 0.03, 0.0, 1, None, 1, 4),
+]
+
+
+@pytest.mark.parametrize("case", EDGE_PARAMS, ids=[c[0] for c in EDGE_PARAMS])
+def test_replay_fp64_edge_parameters(ope, ora, case):
+    # |rho| = 1 (no independent diffusion: sqrt(1 - rho^2) = 0, QE's K3 = K4
+    # = 0), a negative rate with q above r, a 1e-3-year expiry in 3 steps,
+    # kappa dt = 12.5 (QE's e^{-kappa dt} ~ 4e-6): per-path parity as above
+    _, over, expiry, rate, q, scheme, lam, kind, n_steps = case
+    hp = dict(HP, **over)
+    pay_ref, st_ref, run = _oracle_and_gpu(ope, ora, scheme, lam, kind, True, 0, 700, n_steps, hp=hp,
+                                           expiry=expiry, rate=rate, q=q)
+    st = run.path_states
+    for c in (0, 1):
+        np.testing.assert_allclose(st[..., c], st_ref[..., c], rtol=1e-12, atol=1e-300)
+    if kind == 1:
+        np.testing.assert_allclose(st[..., 2], st_ref[..., 2], rtol=1e-12)
+    tol = 1e-12 * np.maximum(np.abs(pay_ref), S0)
+    bad = np.abs(run.path_payoffs - pay_ref) > tol
+    if kind == 2:
+        np.testing.assert_allclose(st[..., 3], st_ref[..., 3], rtol=1e-12)
+        bad &= ~(np.abs(st_ref[..., 3] - 130.0) <= 1e-12 * 130.0)
+    assert not bad.any(), (np.argwhere(bad)[:5], run.path_payoffs[bad][:5], pay_ref[bad][:5])
+    price, se = _ref_price(pay_ref, True)
+    assert abs(run.result.price - price) <= 1e-12 * abs(price) + 1e-15
+    assert abs(run.result.std_error - se) <= 1e-12 * se + 1e-15
