@@ -23,7 +23,8 @@
 //   many workgroups wait.  A worker leaves once draining is set and its
 //   ticket is at or beyond the final tail: every item below that tail was
 //   taken by a worker that runs it, and the rest wait for the next launch.
-// * A worker reads the item and the request's record from host memory with
+// * A worker reads the item and, when the item belongs to another request
+//   than its previous one, the request's record from host memory with
 //   system-scope loads (never through the scalar cache), copies the record
 //   into SGPRs, and runs the same tile body as the batch kernels.
 // * Completing a tile bumps done[slot]; the workgroup that brings it to the
@@ -39,6 +40,8 @@ namespace ope {
 namespace {
 
 constexpr int kRecWords = (int)(sizeof(SrvRecord) / 4);
+constexpr int kRecQwords = kRecWords / 2;
+static_assert(sizeof(SrvRecord) % 8 == 0, "record read in 8-byte words");
 
 // Device-memory words shared by the sentinel and the workers.
 // k_server_prepare sets launch_base = next_base and clears tickets and
@@ -101,9 +104,11 @@ __device__ void sentinel(const SrvLaunch& L, SrvMirror* mirror) {
 template <class Real>
 __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinnedWavesServerF32 : 1)) void k_server(SrvLaunch L, SrvMirror* mirror) {
     __shared__ double2 lds[kTileLds];
-    __shared__ uint32_t s_rec[kRecWords];
+    __shared__ uint64_t s_rec[kRecQwords];
     __shared__ SrvItem s_item;
     __shared__ uint32_t s_status;  // 0 work, 1 leave
+    __shared__ uint32_t s_reload;  // the item belongs to another request than the record in s_rec
+    __shared__ uint64_t s_key;     // request of the record in s_rec
     __shared__ uint32_t s_last;
     __shared__ double s_strike;
     __shared__ double2 s_w[kBlock / 64];
@@ -114,6 +119,7 @@ __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinned
     }
     // set by k_server_prepare, which ran before this launch
     const uint64_t base = __hip_atomic_load(&mirror->launch_base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) s_key = ~0ull;
     for (;;) {
         if (threadIdx.x == 0) {
             const uint64_t ticket =
@@ -145,19 +151,34 @@ __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinned
             }
             s_item = it;
             s_status = status;
+            // target - n_tiles is the slot's cumulative tile count before the
+            // request was admitted: constant over the request's items and
+            // strictly larger for the next request in the slot, so (slot, it)
+            // names the request, whose record does not change while it runs
+            const uint64_t key = ((uint64_t)it.slot << 32) | (uint32_t)(it.target - it.n_tiles);
+            s_reload = key != s_key;
+            s_key = key;
         }
         __syncthreads();
         if (s_status) return;
         const SrvItem it = s_item;
 
-        // the request's record: host memory -> LDS -> SGPRs
-        const uint32_t* rw = reinterpret_cast<const uint32_t*>(L.recs + it.slot);
-        for (int i = threadIdx.x; i < kRecWords; i += kBlock) s_rec[i] = sys_load32(rw + i);
+        // the request's record: host memory -> LDS -> SGPRs.  It is read from
+        // the host only when the item belongs to another request than the
+        // workgroup's previous item: host reads of one cache line from many
+        // workgroups serialise (a burst of 64 requests of 3906 tiles each ran
+        // at half the batch kernel's rate while every tile re-read its record,
+        // at 0.82 of it with the record kept)
+        if (s_reload) {
+            const uint64_t* rw = reinterpret_cast<const uint64_t*>(L.recs + it.slot);
+            for (int i = threadIdx.x; i < kRecQwords; i += kBlock) s_rec[i] = sys_load64(rw + i);
+        }
         __syncthreads();
         SrvRecord rec;
         uint32_t* dst = reinterpret_cast<uint32_t*>(&rec);
+        const uint32_t* src_w = reinterpret_cast<const uint32_t*>(s_rec);
 #pragma unroll
-        for (int i = 0; i < kRecWords; ++i) dst[i] = __builtin_amdgcn_readfirstlane(s_rec[i]);
+        for (int i = 0; i < kRecWords; ++i) dst[i] = __builtin_amdgcn_readfirstlane(src_w[i]);
         if (threadIdx.x == 0) s_strike = rec.strike;
         __syncthreads();
 
