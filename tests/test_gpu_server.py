@@ -132,6 +132,27 @@ def _served_equals_single_call(ope, q, got, precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+def test_one_slot_reused_by_successive_requests(ope, precision):
+    # A workgroup keeps the record of its previous item's request and reads
+    # the record again only for another request.  Through one slot, twelve
+    # different requests follow each other (each adaptive, so a request's
+    # later rounds reuse the record its earlier rounds loaded, while every
+    # new request in the slot must load its own): each result equals the
+    # one-shot call at the path count served, bit for bit.
+    reqs = uneven_requests(ope, 12, 41, max_paths=200_000)
+    print(f"one slot, {len(reqs)} requests, precision={precision}", flush=True)
+    with ope.Server(precision, max_requests=1, idle_exit_ms=50.0, initial_round_pairs=4096) as srv:
+        tickets = [srv.submit(q, 1e-9) for q in reqs]
+        got = [srv.wait(t, WAIT_MS) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["peak_active"] == 1
+    assert sum(g.rounds for g in got) > len(reqs)
+    for q, g in zip(reqs, got):
+        assert g.slot == 0
+        assert _served_equals_single_call(ope, q, g, precision), (q, g)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_adaptive_requests_whose_caps_exceed_the_ring(ope, precision):
     # Six adaptive requests of 30 tiles each (15360 paths) against a 64-tile
     # ring: a request reserves its whole cap at admission, so at most two are
