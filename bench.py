@@ -188,6 +188,10 @@ def serve(ope, precision, rate, seconds, target, seed=5):
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
             "path_steps_per_s": ps / wall, "wall_s": wall,
             "mean_rounds": float(np.mean([g.rounds for g in got])),
+            "requests_by_rounds": {str(k): int(v) for k, v in zip(*np.unique([g.rounds for g in got],
+                                                                             return_counts=True))},
+            "stopped_at_target": int(sum(1 for g in got if g.stop_reason == ope.SERVER_STOP_TARGET)),
+            "paths_over_cap": float(sum(g.paths for g in got) / sum(q.cfg.n_paths for q in reqs[:len(got)])),
             "met_target": int(sum(1 for g in got if g.result.std_error <= target * abs(g.result.price))),
             "peak_active": st["peak_active"], "peak_resident_bytes": st["peak_resident_bytes"],
             "kernel_launches": st["kernel_launches"], "workgroups": st["workgroups"]}
@@ -427,9 +431,17 @@ def serving(ope, seconds):
             srv[name + tag] = r
             print(f"# serving {name}{tag}: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms "
                   f"{r['path_steps_per_s']:.3g} path-steps/s", file=sys.stderr, flush=True)
+    # a looser target that most requests reach before their cap, so that the
+    # adaptive rounds (stop at the target, or post the tiles the SE asks for)
+    # decide most results
+    r = serve(ope, 1, 2000.0, seconds, 1e-2)
+    srv["fp32_target_1e-2"] = r
+    print(f"# serving fp32 target 1e-2: p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms, met {r['met_target']} "
+          f"of {r['requests']}, mean rounds {r['mean_rounds']:.2f}", file=sys.stderr, flush=True)
     srv["workload"] = ("Poisson arrivals at 2000 req/s, requests drawn as the uneven-batch configuration, "
                        "each run until std_error <= 1e-3 x price (cap: its drawn path count); _burst: the "
-                       "same number of requests submitted at once")
+                       "same number of requests submitted at once; fp32_target_1e-2: the arrivals of fp32 "
+                       "with the target 1e-2 x price")
     return srv
 
 
