@@ -50,10 +50,24 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 }
 
 // In the engine's counters (call, pair lo, pair hi, tag) the call index, the
-// pair's high word and the tag are uniform across a wave, so round 0 has
-// uniform multiplicands and round 1's x input is uniform: their xors stay
-// plain (the compiler runs the uniform ones on the scalar unit).  From round
-// 2 on every word is per-lane, and each output word is one bitop3.
+// pair's high word and the tag are uniform across a wave.  Which words are
+// uniform (U) or per-lane (L) entering each round:
+//   round 0 (U, L, U, U), round 1 (L, U, U, U), round 2 (U, U, L, L),
+//   round 3 (L, L, L, U), round 4 on (L, L, L, L).
+// Rounds 0 and 1 keep plain xors (the compiler runs the uniform ones on the
+// scalar unit); from round 2 on each output word is one bitop3.  In round 2
+// both output xors, and in round 3 the z word's, combine one per-lane value
+// with two uniform ones, so their bitop3 copies one scalar operand into a
+// vector register first.  SPLIT xors the uniform pair on the scalar unit
+// and then the per-lane value with one v_xor instead: three vector
+// instructions fewer per call, but more scalar work and scalar-to-vector
+// dependencies.  Same bits either way.  Measured (same box, alternated)
+// with SPLIT in every call: the FP32 Euler kernels 1.2-1.8 % faster (Bates
+// surface 5.16 -> 5.07-5.10 ms, uneven batch 20.3 -> 20.0 ms), the FP32 QE
+// kernels 1 % slower, the FP64 QE kernels 0.5 % slower, FP64 Euler
+// unchanged; so the normal draws of the step loops (philox_steps) split in
+// the FP32 Euler loop only, and the rare jump calls always.
+template <bool SPLIT = true>
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -63,6 +77,10 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         if (r < 2)
             c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        else if (SPLIT && r == 2)
+            c = make_uint4(hi1 ^ (c.y ^ k0), lo1, c.w ^ (hi0 ^ k1), lo0);
+        else if (SPLIT && r == 3)
+            c = make_uint4(xor3(hi1, c.y, k0), lo1, hi0 ^ (c.w ^ k1), lo0);
         else
             c = make_uint4(xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0);
         k0 += kPhiloxW0;
@@ -144,8 +162,9 @@ __device__ __forceinline__ uint32_t poisson_tail(uint32_t w, double mu, double e
 // Philox substream of one pair: key = seed, counter = (call, pair lo, pair hi, tag).
 struct PairStream {
     uint32_t k0, k1, plo, phi;
+    template <bool SPLIT = true>
     __device__ __forceinline__ uint4 call(uint32_t c0, uint32_t tag) const {
-        return philox4x32_10(make_uint4(c0, plo, phi, tag), k0, k1);
+        return philox4x32_10<SPLIT>(make_uint4(c0, plo, phi, tag), k0, k1);
     }
 };
 
@@ -229,6 +248,9 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
     constexpr bool kBlocks = FP32 || !std::is_same<BlockEnd, NoBlockEnd>::value;
     PairStream st = st0;
+    // only the FP32 Euler loop's normal draws split the xors (see
+    // philox4x32_10)
+    constexpr bool kSplit = MIX;
     // DEFER: the lane's recorded events, oldest first, as (step, gate bits);
     // step ~0u marks an empty slot
     uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u, es2 = ~0u, ew2 = 0u;
@@ -284,14 +306,14 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
             if constexpr (MIX) {
 #pragma unroll
                 for (uint32_t k = 0; k < 8; k += 2) {
-                    const uint4 w = st.call((i + k) >> 1, kTagNormal);
+                    const uint4 w = st.template call<kSplit>((i + k) >> 1, kTagNormal);
                     one(w.x, w.y, i + k);
                     one(w.z, w.w, i + k + 1);
                 }
             } else {
 #pragma nounroll
                 for (uint32_t k = 0; k < 8; k += 2) {
-                    const uint4 w = st.call((i + k) >> 1, kTagNormal);
+                    const uint4 w = st.template call<kSplit>((i + k) >> 1, kTagNormal);
                     one(w.x, w.y, i + k);
                     one(w.z, w.w, i + k + 1);
                 }
@@ -300,12 +322,12 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
         }
     }
     for (; i + 2 <= N; i += 2) {
-        const uint4 w = st.call(i >> 1, kTagNormal);
+        const uint4 w = st.template call<kSplit>(i >> 1, kTagNormal);
         one(w.x, w.y, i);
         one(w.z, w.w, i + 1);
     }
     if (i < N) {
-        const uint4 w = st.call(i >> 1, kTagNormal);
+        const uint4 w = st.template call<kSplit>(i >> 1, kTagNormal);
         one(w.x, w.y, i);
     }
     if constexpr (DEFER) {

@@ -44,10 +44,13 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const McDesc* __restrict__ de
     if (threadIdx.x == 0) out[d.out_off + jb.y] = r;
 }
 
+// Odd counters go through the split arrangement of the xors
+// (philox4x32_10<true>), even ones through the other (the QE loops'),
+// so the known-answer checks hold both to the same bits.
 __global__ void k_debug_philox(uint32_t k0, uint32_t k1, const uint4* __restrict__ ctr, uint4* __restrict__ out,
                                uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
+    if (i < n) out[i] = (i & 1u) ? philox4x32_10<true>(ctr[i], k0, k1) : philox4x32_10<false>(ctr[i], k0, k1);
 }
 
 // The draws of the pricing loops (philox_steps, as sim_pair_f64 / sim_pair_f32

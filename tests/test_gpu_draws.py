@@ -22,8 +22,12 @@ def test_philox_known_answers_on_device(ope):
          (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
     ]
     for ctr, key, want in cases:
-        out = ope.debug_philox(key[0] | (key[1] << 32), np.array([ctr], dtype=np.uint32))
+        # rows 0 and 1: the two arrangements of the xors (the device kernel
+        # runs odd rows through the split one, even rows through the QE
+        # loops')
+        out = ope.debug_philox(key[0] | (key[1] << 32), np.array([ctr, ctr], dtype=np.uint32))
         assert tuple(int(x) for x in out[0]) == want
+        assert tuple(int(x) for x in out[1]) == want
 
 
 def test_philox_random_counters_match_oracle(ope, ora):
