@@ -65,8 +65,10 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 // with SPLIT in every call: the FP32 Euler kernels 1.2-1.8 % faster (Bates
 // surface 5.16 -> 5.07-5.10 ms, uneven batch 20.3 -> 20.0 ms), the FP32 QE
 // kernels 1 % slower, the FP64 QE kernels 0.5 % slower, FP64 Euler
-// unchanged; so the normal draws of the step loops (philox_steps) split in
-// the FP32 Euler loop only, and the rare jump calls always.
+// unchanged, and the latency-bound Heston Euler call (100k paths, one wave
+// per SIMD, where every dependency shows) 6 % slower; so the normal draws of
+// the step loops (philox_steps) split in the FP32 Bates Euler loop only, and
+// the rare jump calls always.
 template <bool SPLIT = true>
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -248,9 +250,9 @@ __device__ __forceinline__ void philox_steps(const McDesc& d, const PairStream& 
     static_assert(!MIX || FP32, "mixed normals are an FP32 mode");
     constexpr bool kBlocks = FP32 || !std::is_same<BlockEnd, NoBlockEnd>::value;
     PairStream st = st0;
-    // only the FP32 Euler loop's normal draws split the xors (see
+    // only the FP32 Bates Euler loop's normal draws split the xors (see
     // philox4x32_10)
-    constexpr bool kSplit = MIX;
+    constexpr bool kSplit = MIX && JUMPS;
     // DEFER: the lane's recorded events, oldest first, as (step, gate bits);
     // step ~0u marks an empty slot
     uint32_t es0 = ~0u, ew0 = 0u, es1 = ~0u, ew1 = 0u, es2 = ~0u, ew2 = 0u;
