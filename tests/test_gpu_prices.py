@@ -317,3 +317,25 @@ def test_fp32_tracks_fp64_at_edge_parameters(ope, case):
         assert math.isfinite(r32.price) and math.isfinite(r32.std_error), (r32, r64)
         # (the 95 put at the 1e-3-year expiry is ~8 sd out of the money: both 0)
         assert abs(r32.price - r64.price) <= 0.1 * r64.std_error + 1e-12, (case[0], pay, r32, r64)
+
+
+@pytest.mark.parametrize("kind", ["asian", "uao"])
+def test_gpu_matches_cpu_oracle_at_full_size_qe_1m_x_365(ope, ora, kind):
+    # The headline configuration at its full size (Heston QE, 1M paths x 365
+    # steps, arithmetic Asian call and up-and-out call B=130), both
+    # precisions against the CPU oracle's mc_price at the same path count:
+    # independent draws of the same scheme, so within 3 combined SE.
+    import os
+
+    hp = dict(v0=0.04, kappa=1.5, theta=0.04, sigma=0.5, rho=-0.7)
+    spec = ("A", 100.0) if kind == "asian" else ("U", 100.0, 130.0)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    print(f"oracle QE {kind} 1M x 365 on {threads} threads", flush=True)
+    c = ora.mc_price(100.0, 1.0, 0.03, 0.0, ora.heston(**hp), _ora_payoff(ora, spec), 1_000_000, 365, 42,
+                     True, 1, None, n_threads=threads)
+    cfg = ope.McConfig(n_paths=1_000_000, n_steps=365, seed=42, scheme=1)
+    for precision in PRECISIONS:
+        g = run(ope, precision, _payoff(ope, spec), cfg, hp=hp, rate=0.03).result
+        print(f"precision={precision} gpu={g} oracle=({c[0]}, {c[1]})", flush=True)
+        assert abs(g.price - c[0]) < 3 * math.hypot(g.std_error, c[1]), (precision, g, c[:2])
+        assert 0.97 < g.std_error / c[1] < 1.03
