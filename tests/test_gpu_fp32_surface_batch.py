@@ -41,8 +41,12 @@ def test_fp32_within_tenth_se_bates(ope, scheme):
     assert abs(f32.price - f64.price) <= 0.1 * f64.std_error
 
 
-def test_fp32_surface_within_tenth_se_full_size(ope):
-    # Bates, 4M paths x 252 steps, 41 strikes 60..140 at 8 expiries, FP32 vs FP64
+def test_fp32_surface_within_tenth_se_full_size(ope, ora):
+    # Bates, 4M paths x 252 steps, 41 strikes 60..140 at 8 expiries, FP32 vs
+    # FP64; two of the entries also against the CPU oracle's mc_bates at the
+    # same path count (independent draws: within 3 combined SE)
+    import os
+
     strikes = np.arange(60.0, 140.0 + 1e-9, 2.0)
     expiries = [0.1, 0.25, 0.5, 0.75, 1.0, 1.25, 1.5, 2.0]
     cfg = ope.McConfig(n_paths=4_000_000, n_steps=252, seed=42)
@@ -53,6 +57,16 @@ def test_fp32_surface_within_tenth_se_full_size(ope):
     print(f"surface max |fp32-fp64|/se = {err.max():.4f}; fp64 {ms64:.2f} ms, fp32 {ms32:.2f} ms", flush=True)
     assert err.max() <= 0.1
     assert st64[0].path_steps == 4_000_000 * 252
+    threads = max(1, min(16, os.cpu_count() or 1))
+    for ie, ik in ((4, 20), (1, 15)):   # (T=1, K=100), (T=0.25, K=90)
+        T, K = expiries[ie], float(strikes[ik])
+        print(f"oracle Bates T={T} K={K} 4M x 252 on {threads} threads", flush=True)
+        c = ora.mc_price(100.0, T, 0.03, 0.0, ora.heston(**BASE), ora.payoff(ora.EUROPEAN, K), 4_000_000, 252, 42,
+                         True, 0, ora.jumps(*JUMP), n_threads=threads)
+        for p, se in ((p64, s64), (p32, s32)):
+            print(f"  gpu {p[ie, ik]} +- {se[ie, ik]}, oracle {c[0]} +- {c[1]}", flush=True)
+            assert abs(p[ie, ik] - c[0]) < 3 * math.hypot(se[ie, ik], c[1])
+            assert 0.97 < se[ie, ik] / c[1] < 1.03
 
 
 @pytest.mark.parametrize("precision", [0, 1])
