@@ -443,11 +443,11 @@ def mc_batch(requests: Sequence[McRequest], precision=PRECISION_FP64, with_kerne
     """All requests in one kernel launch.  Returns (results, stats, device_ms),
     plus the path-kernel time when with_kernel_ms."""
     n = len(requests)
-    arr = np.zeros(max(n, 1), _REQUEST_NP)
-    arr[:n] = [r._row() for r in requests]
+    arr = (_Request * max(n, 1))()   # zeroed; filled through a numpy view of its bytes
+    np.frombuffer(arr, _REQUEST_NP)[:n] = [r._row() for r in requests]
     h = _OptionsHolder(McOptions(precision), max(n, 1))
     out = (_Result * max(n, 1))()
-    _check(lib().ope_mc_batch(arr.ctypes.data_as(C.POINTER(_Request)), n, C.byref(h.c), out))
+    _check(lib().ope_mc_batch(arr, n, C.byref(h.c), out))
     v = np.frombuffer(out, _RESULT_NP)[:n]
     res = [McResult(p, s) for p, s in zip(v["price"].tolist(), v["std_error"].tolist())]
     if with_kernel_ms:
