@@ -153,8 +153,9 @@ __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinned
             s_status = status;
             // target - n_tiles is the slot's cumulative tile count before the
             // request was admitted: constant over the request's items and
-            // strictly larger for the next request in the slot, so (slot, it)
-            // names the request, whose record does not change while it runs
+            // strictly larger for the next request in the slot, so (slot,
+            // target - n_tiles) names the request, whose record does not
+            // change while it runs
             const uint64_t key = ((uint64_t)it.slot << 32) | (uint32_t)(it.target - it.n_tiles);
             s_reload = key != s_key;
             s_key = key;
@@ -167,8 +168,9 @@ __global__ __launch_bounds__(kBlock, (std::is_same<Real, float>::value ? kPinned
         // the host only when the item belongs to another request than the
         // workgroup's previous item: host reads of one cache line from many
         // workgroups serialise (a burst of 64 requests of 3906 tiles each ran
-        // at half the batch kernel's rate while every tile re-read its record,
-        // at 0.82 of it with the record kept)
+        // at half the batch kernel's rate while every tile re-read its record
+        // in 4-byte words, at 0.95 of it with the record kept and read in
+        // 8-byte words)
         if (s_reload) {
             const uint64_t* rw = reinterpret_cast<const uint64_t*>(L.recs + it.slot);
             for (int i = threadIdx.x; i < kRecQwords; i += kBlock) s_rec[i] = sys_load64(rw + i);
