@@ -34,7 +34,9 @@ struct Code {
 // bodies at 3 waves (unpinned the compiler spends 169-177 VGPRs where 129-163
 // suffice without spilling, and the QE pair runs 3.63 ms at 3 waves against
 // 4.04 ms at 2) and the Bates Euler bodies at 5 (16.70 against 16.80 ms for
-// the surface).  1: no constraint.
+// the surface unpinned; at 6 they fit 80 VGPRs by spilling 28-80 bytes per
+// lane and run 16.1 against 15.8 ms for the surface, 70.0 against 60.4 ms
+// for the uneven batch).  1: no constraint.
 template <class Real, int CODE>
 constexpr int pinned_waves() {
     if (!std::is_same<Real, float>::value) {
