@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -34,6 +35,8 @@ struct CfContext {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void* buf = nullptr;  // descriptors, then prices, then stats
     size_t cap = 0;
+    void* host = nullptr;  // pinned staging laid out as buf
+    size_t host_cap = 0;
 };
 
 CfContext g_cf;
@@ -91,6 +94,19 @@ int ensure_buf(CfContext& c, size_t need) {
     return OPE_OK;
 }
 
+// The pinned staging buffer, grown to at least need bytes: copies from and
+// to pinned memory run as single DMA transfers instead of the runtime's
+// chunked staging of pageable memory.  Called with g_cf.mu held.
+int ensure_host(CfContext& c, size_t need) {
+    if (need <= c.host_cap) return OPE_OK;
+    if (c.host) HIP_TRY(hipHostFree(c.host));
+    c.host = nullptr;
+    c.host_cap = 0;
+    HIP_TRY(hipHostMalloc(&c.host, need, hipHostMallocDefault));
+    c.host_cap = need;
+    return OPE_OK;
+}
+
 int elapsed(CfContext& c, double* kernel_ms) {
     if (kernel_ms) {
         float ms = 0.0f;
@@ -112,29 +128,33 @@ int run_cf(const std::vector<CfDesc>& descs, double* out, uint32_t* panels, doub
     CfContext& c = g_cf;
     const size_t b_desc = sizeof(CfDesc) * n, b_out = sizeof(double) * n, b_stats = sizeof(CfStats) * n;
     const size_t b_dout = dout ? 3 * sizeof(double) * n : 0;
-    if (int rc = ensure_buf(c, b_desc + b_out + b_stats + b_dout)) return rc;
+    const size_t total = b_desc + b_out + b_stats + b_dout;
+    if (int rc = ensure_buf(c, total)) return rc;
+    if (int rc = ensure_host(c, total)) return rc;
     char* base = static_cast<char*>(c.buf);
+    char* hbase = static_cast<char*>(c.host);
     CfDesc* d_desc = reinterpret_cast<CfDesc*>(base);
     double* d_out = reinterpret_cast<double*>(base + b_desc);
     CfStats* d_stats = reinterpret_cast<CfStats*>(base + b_desc + b_out);
     double* d_dout = dout ? reinterpret_cast<double*>(base + b_desc + b_out + b_stats) : nullptr;
-    HIP_TRY(hipMemcpyAsync(d_desc, descs.data(), b_desc, hipMemcpyHostToDevice, c.stream));
+    std::memcpy(hbase, descs.data(), b_desc);
+    HIP_TRY(hipMemcpyAsync(d_desc, hbase, b_desc, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(hipEventRecord(c.ev0, c.stream));
     HIP_TRY(launch_cf(d_desc, (uint32_t)n, d_out, panels ? d_stats : nullptr, d_dout, c.stream));
     HIP_TRY(hipEventRecord(c.ev1, c.stream));
-    HIP_TRY(hipMemcpyAsync(out, d_out, b_out, hipMemcpyDeviceToHost, c.stream));
-    if (dout) HIP_TRY(hipMemcpyAsync(dout, d_dout, b_dout, hipMemcpyDeviceToHost, c.stream));
-    std::vector<CfStats> st;
-    if (panels) {
-        st.resize(n);
-        HIP_TRY(hipMemcpyAsync(st.data(), d_stats, b_stats, hipMemcpyDeviceToHost, c.stream));
-    }
+    // Prices, stats and derivatives in one transfer (the stats bytes are
+    // unwritten when no panel counts were asked for, and unread then).
+    HIP_TRY(hipMemcpyAsync(hbase + b_desc, d_out, total - b_desc, hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
-    if (panels)
+    std::memcpy(out, hbase + b_desc, b_out);
+    if (dout) std::memcpy(dout, hbase + b_desc + b_out + b_stats, b_dout);
+    if (panels) {
+        const CfStats* st = reinterpret_cast<const CfStats*>(hbase + b_desc + b_out);
         for (size_t i = 0; i < n; ++i) {
             panels[2 * i] = st[i].panels_p1;
             panels[2 * i + 1] = st[i].panels_p2;
         }
+    }
     return elapsed(c, kernel_ms);
 }
 
